@@ -1,0 +1,465 @@
+// solver.cpp -- the time step of libcdiff (PseudoTransientScheme.step!,
+// src/pseudo_transient/PseudoTransientSolver.jl:180-351) on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "solver.h"
+#include "sweep_args.h"
+
+using namespace cdiff;
+
+#define HIP_TRY(expr)                                  \
+  do {                                                 \
+    hipError_t _e = (expr);                            \
+    if (_e != hipSuccess) return hip_fail(_e, #expr);  \
+  } while (0)
+
+#define LAUNCH_CHECK(name)                                          \
+  do {                                                              \
+    hipError_t _e = hipGetLastError();                              \
+    if (_e != hipSuccess) return hip_fail(_e, "launch of " name);   \
+  } while (0)
+
+namespace {
+
+dim3 grid_full(const Geo& g) {
+  const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  return dim3((unsigned)((n + 255) / 256));
+}
+
+// The error-check schedule of PseudoTransientSolver.jl:206-208 and :280:
+// m = round(0.9 N) (ties to even), k = max(1, (N - m) / 2).
+void schedule(int last, int& m, int& k) {
+  m = (int)std::nearbyint((double)last * 0.9);
+  k = std::max(1, (last - m) / 2);
+}
+
+inline bool is_check(int it, int m, int k) { return (it >= m && it % k == 0) || it == 1; }
+
+int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  return std::atoi(v);
+}
+
+}  // namespace
+
+void cdiff_solver::configure_sweep() {
+  const int ni = g.hi[0] - g.lo[0] + 1;
+  const int nj = g.hi[1] - g.lo[1] + 1;
+  if (ndim == 2) {
+    strips = (ni + OWN - 1) / OWN;
+    int c = env_int("CDIFF_CH", 0);
+    if (c <= 0) {
+      // aim for ~4096 waves (16 per CU), rows per chunk between 8 and 64
+      const int64_t cells = (int64_t)strips * nj;
+      c = (int)std::min<int64_t>(64, std::max<int64_t>(8, (cells + 4095) / 4096));
+    }
+    ch = std::min(c, nj);
+    chunks = (nj + ch - 1) / ch;
+    ntiles = strips * chunks;
+    nblocks = (ntiles + 7) / 8 * 8;
+  } else {
+    const int nk = g.hi[2] - g.lo[2] + 1;
+    strips3[0] = (ni + 15 - 2) / (16 - 2);
+    strips3[1] = (nj + 16 - 2 - 1) / (16 - 2);
+    int c = env_int("CDIFF_CH3", 0);
+    if (c <= 0) c = std::min(nk, 32);
+    ch = c;
+    chunks = (nk + ch - 1) / ch;
+    ntiles = strips3[0] * strips3[1] * chunks;
+    nblocks = (ntiles + 7) / 8 * 8;
+  }
+  nparts = nblocks;
+}
+
+int cdiff_solver::enqueue_next_dt(const double* unew, const double* uold, double dt, double tol, double muc,
+                                  double growth) {
+  double* part = red + 8;
+  const int nb = 1024;
+  hipLaunchKernelGGL(k_next_dt_partial, dim3(nb), dim3(256), 0, stream, g, unew, uold, (const double*)red, tol, 0,
+                     part);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, stream, (const double*)part, nb, red + 0);
+  hipLaunchKernelGGL(k_next_dt_partial, dim3(nb), dim3(256), 0, stream, g, unew, uold, (const double*)red, tol, 1,
+                     part);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, stream, (const double*)part, nb, red + 1);
+  hipLaunchKernelGGL(k_next_dt_final, dim3(1), dim3(1), 0, stream, (const double*)(red + 1), dt, muc, growth, red + 2);
+  LAUNCH_CHECK("next_dt kernels");
+  return CDIFF_OK;
+}
+
+// Launch the FP64 sweeps of one time step in segments (see cdiff.cpp).
+int cdiff_solver::run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int /*first_iter_base*/) {
+  int m, k;
+  schedule(last_iter, m, k);
+  const int lmin = std::max(1, env_int("CDIFF_SEG_MIN", 4));
+  const int lmax = std::max(lmin, env_int("CDIFF_SEG_MAX", 256));
+  const int64_t limit = (int64_t)o.max_iter + 1;
+
+  Sweep2DArgs<double> A{};
+  A.g = g;
+  A.gh = gh;
+  A.kap = kap;
+  A.ic.L = L;
+  A.ic.Vpdtau = Vpdtau;
+  A.ic.dt = dt;
+  A.harmonic = mean == CDIFF_MEAN_HARMONIC;
+  A.ntiles = ntiles;
+  A.chunks = chunks;
+  A.ch = ch;
+  A.u_prev = u_prev;
+  A.g11 = coef[0];
+  A.g22 = coef[1];
+  A.c12 = coef[2];
+  A.ac = coef[3];
+  A.bc = coef[4];
+  A.S = S;
+  A.rho = rho;
+  A.tzA = tzA;
+  A.tzB = tzB;
+  if (!o.subcycle_conductivity) {
+    A.frozen_alpha = frozen[0];
+    A.frozen_theta = frozen[1];
+    A.frozen_dtau = frozen[2];
+  }
+  A.part_sum = part_sum;
+  A.part_max = part_max;
+  A.ctrl = ctrl;
+
+  int64_t it = 0;
+  int seg = 0;
+  bool pending = false;
+  while (true) {
+    const int64_t s = it + 1;
+    const int64_t cap = std::min<int64_t>(s + lmax - 1, limit);
+    int64_t e = std::min<int64_t>(s + lmin - 1, cap);
+    while (e < cap && !is_check((int)e, m, k)) ++e;
+    for (int64_t t = s; t <= e; ++t) {
+      const bool chk = is_check((int)t, m, k);
+      const int src = (int)((cur + t - 1) & 1), dst = 1 - src;
+      if (ndim == 2) {
+        A.u_src = u[src];
+        A.u_dst = u[dst];
+        A.qx_src = q[0][src];
+        A.qx_dst = q[0][dst];
+        A.qy_src = q[1][src];
+        A.qy_dst = q[1][dst];
+        launch_sweep2d<double>(A, chk, nblocks, stream);
+        LAUNCH_CHECK("sweep2d");
+      } else {
+        int rc = launch3d_fp64(o, dt, Vpdtau, src, chk);
+        if (rc) return rc;
+      }
+      if (chk || t == limit) {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, ctrl, (const double*)part_sum,
+                           (const double*)part_max, chk ? nparts : 0, (int)t, o.max_iter, o.rtol, o.atol, N_full,
+                           o.norm_kind, chk ? 1 : 0);
+        LAUNCH_CHECK("k_finalize");
+      }
+    }
+    it = e;
+    HIP_TRY(hipMemcpyAsync(&ctrl_h[seg & 1], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipEventRecord(ev[seg & 1], stream));
+    if (pending) {
+      HIP_TRY(hipEventSynchronize(ev[(seg - 1) & 1]));
+      if (ctrl_h[(seg - 1) & 1].done) break;
+    }
+    if (it >= limit) break;
+    pending = true;
+    ++seg;
+  }
+  HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  return CDIFF_OK;
+}
+
+int cdiff_solver::validation_message(int status) {
+  const bool is_alpha = status == CDIFF_INVALID_ALPHA;
+  const double* dev = is_alpha ? alpha : S;
+  const char* name = is_alpha ? "diffusivity" : "source_term";
+  const int positivity = is_alpha ? enforce_pos_last : 0;
+  const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  std::vector<double> a(n, 0.0);
+  if (dev) {
+    hipError_t e = hipMemcpy2D(a.data(), sizeof(double) * g.n[0], dev + g.off, sizeof(double) * g.pitch,
+                               sizeof(double) * g.n[0], (size_t)g.n[1] * g.n[2], hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy2D(validate)");
+  }
+  auto idx = [&](int i, int j, int k) { return (int64_t)i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k); };
+  auto bad = [&](double v) { return !std::isfinite(v) || (positivity && !(v >= 0)); };
+  bool dom_bad = false, any_nan = false;
+  double mn = INFINITY, mx = -INFINITY;
+  for (int k = g.lo[2]; k <= g.hi[2]; ++k)
+    for (int j = g.lo[1]; j <= g.hi[1]; ++j)
+      for (int i = g.lo[0]; i <= g.hi[0]; ++i) {
+        const double v = a[idx(i, j, k)];
+        dom_bad |= bad(v);
+        any_nan |= std::isnan(v);
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+      }
+  char buf[256];
+  if (dom_bad) {
+    if (any_nan) mn = mx = NAN;
+    std::snprintf(buf, sizeof buf, "Invalid %s in the domain extrema -> (%g, %g)", name, mn, mx);
+    err = buf;
+    return status;
+  }
+  int flo[3], fhi[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    flo[ax] = ax < ndim ? g.lo[ax] - 1 : 0;
+    fhi[ax] = ax < ndim ? g.hi[ax] + 1 : 0;
+  }
+  for (int ax = 0; ax < ndim; ++ax)
+    for (int side = 0; side < 2; ++side) {
+      const int plane = side ? fhi[ax] : flo[ax];
+      bool b = false;
+      for (int k = flo[2]; k <= fhi[2]; ++k)
+        for (int j = flo[1]; j <= fhi[1]; ++j)
+          for (int i = flo[0]; i <= fhi[0]; ++i) {
+            const int c[3] = {i, j, k};
+            if (c[ax] == plane && bad(a[idx(i, j, k)])) b = true;
+          }
+      if (b) {
+        std::snprintf(buf, sizeof buf, "Invalid %s in the %s halo region for axis: %d", name, side ? "hi" : "lo",
+                      ax + 1);
+        err = buf;
+        return status;
+      }
+    }
+  err = std::string("Invalid ") + name;
+  return status;
+}
+
+int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, double* next_dt_out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  t_written = false;
+  if (!metrics_set) return fail(CDIFF_NOT_READY, "cdiff_set_metrics has not been called");
+  if (!(dt > 0)) return fail(CDIFF_BAD_ARG, "AssertionError: dt > 0");
+  double smin = spacing[0];
+  for (int a = 0; a < ndim; ++a) {
+    if (!(spacing[a] > 0)) return fail(CDIFF_BAD_ARG, "AssertionError: all(solver.spacing .> 0)");
+    smin = std::min(smin, spacing[a]);
+  }
+  const double Vpdtau = o.cfl * smin;
+  if (!(Vpdtau > 0)) return fail(CDIFF_BAD_ARG, "AssertionError: Vpdτ > 0");
+  if (o.max_iter < 0) return fail(CDIFF_BAD_ARG, "max_iter must be >= 0");
+  if (o.norm_kind != 0 && o.norm_kind != 1) return fail(CDIFF_BAD_ARG, "norm_kind must be 0 or 1");
+  enforce_pos_last = o.enforce_positivity;
+  last_dt = dt;
+  last_Vpdtau = Vpdtau;
+  last_subcycle = o.subcycle_conductivity;
+  cdiff_stats st{};
+
+  // prologue: u = u_prev = T (PseudoTransientSolver.jl:229-230), validity (:238-250)
+  for (double* d : {u[0], u[1], u_prev}) {
+    hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)Tdev, d);
+    LAUNCH_CHECK("k_copy");
+  }
+  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, o.rtol, o.atol);
+  hipLaunchKernelGGL(k_validate, grid_full(g), dim3(256), 0, stream, g, (const double*)alpha, o.enforce_positivity, 0,
+                     ctrl);
+  if (S) hipLaunchKernelGGL(k_validate, grid_full(g), dim3(256), 0, stream, g, (const double*)S, 0, 16, ctrl);
+  hipLaunchKernelGGL(k_ctrl_validated, dim3(1), dim3(1), 0, stream, ctrl);
+  LAUNCH_CHECK("prologue");
+  if (dirichlet_outer)
+    for (int b = 0; b < 2; ++b) {
+      hipLaunchKernelGGL(k_halo_fill, grid_full(g), dim3(256), 0, stream, g, gh, u[b], 1);
+      LAUNCH_CHECK("k_halo_fill");
+    }
+
+  if (o.precision == CDIFF_PREC_FP32_DC) {
+    int rc = step_fp32_dc(dt, o, Vpdtau, &st);
+    if (rc) return rc;
+  } else if (o.precision == CDIFF_PREC_FP64) {
+    if (!o.subcycle_conductivity) {
+      // iteration-0 alpha and parameters from the BC-applied u (PseudoTransientSolver.jl:254-261)
+      for (int t = 0; t < 3; ++t)
+        if (!frozen[t]) {
+          HIP_TRY(hipMalloc((void**)&frozen[t], sizeof(double) * g.total));
+          owned.push_back(frozen[t]);
+          hipLaunchKernelGGL(k_fill, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, 0.0, frozen[t]);
+        }
+      hipLaunchKernelGGL(k_halo_fill, grid_full(g), dim3(256), 0, stream, g, gh, u[cur], 0);
+      hipLaunchKernelGGL(k_alpha_full, grid_full(g), dim3(256), 0, stream, g, kap, (const double*)rho,
+                         (const double*)nullptr, (const double*)u[cur], frozen[0]);
+      IterConst ic{L, Vpdtau, dt};
+      hipLaunchKernelGGL(k_iter_params_full, grid_full(g), dim3(256), 0, stream, g, ic, (const double*)rho,
+                         (const double*)frozen[0], frozen[2], frozen[1]);
+      LAUNCH_CHECK("frozen parameters");
+    }
+    int rc = run_loop64(o, dt, Vpdtau, 0);
+    if (rc) return rc;
+  } else {
+    return fail(CDIFF_BAD_ARG, "unknown precision mode");
+  }
+
+  const Ctrl& c = ctrl_h[2];
+  if (o.precision == CDIFF_PREC_FP64) {
+    st.rel_err = c.rel;
+    st.abs_err = c.abs_;
+    st.niter = c.final_iter;
+    st.nchecks = c.nchecks;
+    st.outer = 0;
+  }
+  const int status = c.status;
+  if (status != CDIFF_OK) {
+    if (stats) *stats = st;
+    char buf[256];
+    if (status == CDIFF_NONFINITE) {
+      std::snprintf(buf, sizeof buf, "Non-finite error detected! abs_error = %g, rel_error = %g, exiting...", c.abs_,
+                    c.rel);
+      return fail(status, buf);
+    }
+    if (status == CDIFF_MAX_ITER) {
+      std::snprintf(buf, sizeof buf,
+                    "Maximum iteration limit reached (%d), abs_error = %g, rel_error = %g, exiting...", o.max_iter,
+                    c.abs_, c.rel);
+      return fail(status, buf);
+    }
+    return validation_message(status);
+  }
+  const int fi = c.final_iter;
+  const int src = (int)((cur + fi - 1) & 1), dst = 1 - src;
+  last_iter = fi;
+  // alpha of the last iteration (for the next step's validate_scalar, :238)
+  if (o.subcycle_conductivity || o.precision != CDIFF_PREC_FP64) {
+    hipLaunchKernelGGL(k_alpha_full, grid_full(g), dim3(256), 0, stream, g, kap, (const double*)rho,
+                       (const double*)u[src], (const double*)u[dst], alpha);
+  } else {
+    hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)frozen[0],
+                       alpha);
+  }
+  hipLaunchKernelGGL(k_epilogue_u, grid_full(g), dim3(256), 0, stream, g, (const double*)u[dst], Tdev,
+                     o.enforce_positivity, o.apply_cutoff);
+  LAUNCH_CHECK("epilogue");
+  if (o.calculate_next_dt) {
+    int rc = enqueue_next_dt(Tdev, u_prev, dt, o.tol, o.maximum_u_change, o.timestep_growth_factor);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(red_h, red + 2, sizeof(double), hipMemcpyDeviceToHost, stream));
+  }
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (next_dt_out) *next_dt_out = o.calculate_next_dt ? red_h[0] : dt;
+  cur = dst;
+  last_src = src;
+  have_last = true;
+  t_written = true;
+  const auto t1 = std::chrono::steady_clock::now();
+  st.time = std::chrono::duration<double>(t1 - t0).count();
+  if (stats) *stats = st;
+  return CDIFF_OK;
+}
+
+int cdiff_solver::get_field(int field, double* out) {
+  const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  auto dl = [&](const double* d) -> int {
+    HIP_TRY(hipMemcpy2DAsync(out, sizeof(double) * g.n[0], d + g.off, sizeof(double) * g.pitch,
+                             sizeof(double) * g.n[0], (size_t)g.n[1] * g.n[2], hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return CDIFF_OK;
+  };
+  switch (field) {
+    case CDIFF_FIELD_U: return dl(Tdev);
+    case CDIFF_FIELD_U_PREV: return dl(u_prev);
+    case CDIFF_FIELD_ALPHA: return dl(alpha);
+    case CDIFF_FIELD_SOURCE:
+      if (S) return dl(S);
+      std::fill(out, out + n, 0.0);
+      return CDIFF_OK;
+    case CDIFF_FIELD_DENSITY:
+      if (rho) return dl(rho);
+      std::fill(out, out + n, 1.0);
+      return CDIFF_OK;
+    default: break;
+  }
+  if (field >= CDIFF_FIELD_Q && field < CDIFF_FIELD_Q + ndim) return dl(q[field - CDIFF_FIELD_Q][cur]);
+  if (field >= CDIFF_FIELD_CACHE && field < CDIFF_FIELD_CACHE + ndim)
+    return dl(coef[(ndim == 2 ? 3 : 6) + field - CDIFF_FIELD_CACHE]);
+  const bool want_params = field == CDIFF_FIELD_DTAU_RHO || field == CDIFF_FIELD_THETA_R_DTAU;
+  const bool want_qp = field >= CDIFF_FIELD_QPRIME && field < CDIFF_FIELD_QPRIME + ndim;
+  const bool want_res = field == CDIFF_FIELD_RES;
+  if (!want_params && !want_qp && !want_res) return fail(CDIFF_BAD_ARG, "unknown field");
+  // scratch: merged pre-update iterate, its alpha, and the plain fluxes
+  std::vector<double*> tmp;
+  auto scratch = [&]() -> double* {
+    double* p = nullptr;
+    if (hipMalloc((void**)&p, sizeof(double) * g.total) != hipSuccess) return nullptr;
+    hipLaunchKernelGGL(k_fill, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, 0.0, p);
+    tmp.push_back(p);
+    return p;
+  };
+  auto release = [&]() {
+    (void)hipStreamSynchronize(stream);
+    for (double* p : tmp) (void)hipFree(p);
+  };
+  int rc = CDIFF_OK;
+  if (want_params) {
+    double* d = scratch();
+    double* t = scratch();
+    if (!d || !t) {
+      release();
+      return fail(CDIFF_HIP_ERROR, "hipMalloc(scratch)");
+    }
+    IterConst ic{L, last_Vpdtau, last_dt};
+    hipLaunchKernelGGL(k_iter_params_full, grid_full(g), dim3(256), 0, stream, g, ic, (const double*)rho,
+                       (const double*)alpha, d, t);
+    rc = dl(field == CDIFF_FIELD_DTAU_RHO ? d : t);
+    release();
+    return rc;
+  }
+  if (!have_last) {
+    // nothing computed yet: q' and res are still the constructor's zeros
+    std::fill(out, out + n, 0.0);
+    return CDIFF_OK;
+  }
+  const int src = last_src, dst = 1 - last_src;
+  double* qp[3] = {nullptr, nullptr, nullptr};
+  for (int a = 0; a < ndim; ++a) qp[a] = scratch();
+  double* a_pre = scratch();
+  double* res = scratch();
+  for (double* p : tmp)
+    if (!p) {
+      release();
+      return fail(CDIFF_HIP_ERROR, "hipMalloc(scratch)");
+    }
+  // merged pre-update iterate: domain from u[src], ghosts from u[dst]
+  double* merged = scratch();
+  if (!merged) {
+    release();
+    return fail(CDIFF_HIP_ERROR, "hipMalloc(scratch)");
+  }
+  hipLaunchKernelGGL(k_merge, grid_full(g), dim3(256), 0, stream, g, (const double*)u[src], (const double*)u[dst],
+                     merged);
+  if (last_subcycle)
+    hipLaunchKernelGGL(k_alpha_full, grid_full(g), dim3(256), 0, stream, g, kap, (const double*)rho,
+                       (const double*)nullptr, (const double*)merged, a_pre);
+  else
+    hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)alpha,
+                       a_pre);
+  for (int a = 0; a < ndim; ++a)
+    hipLaunchKernelGGL(k_plain_flux, grid_full(g), dim3(256), 0, stream, g, a, (int)(mean == CDIFF_MEAN_HARMONIC),
+                       (const double*)merged, (const double*)a_pre, qp[a]);
+  if (want_qp) {
+    rc = dl(qp[field - CDIFF_FIELD_QPRIME]);
+    release();
+    return rc;
+  }
+  Coefs cf{};
+  for (int t = 0; t < 9; ++t) cf.c[t] = coef[t];
+  hipLaunchKernelGGL(k_residual_field, grid_full(g), dim3(256), 0, stream, g, last_dt, (const double*)u[dst],
+                     (const double*)u_prev, (const double*)qp[0], (const double*)qp[1],
+                     (const double*)(ndim == 3 ? qp[2] : qp[0]), cf, (const double*)S, (const double*)tzA,
+                     (const double*)tzB, (const double*)a_pre, res);
+  rc = dl(res);
+  release();
+  return rc;
+}

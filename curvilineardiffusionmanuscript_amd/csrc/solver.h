@@ -1,0 +1,82 @@
+// solver.h -- the libcdiff handle (struct cdiff_solver) and its methods.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cdiff.h"
+#include "cdiff_dev.h"
+
+struct cdiff_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  cdiff::Geo g{};
+  int ndim = 2, nhalo = 1, mean = 0;
+  double cp = 1.0;
+  cdiff_bc bcs[6]{};
+  cdiff::Ghost gh{};
+  bool dirichlet_outer = false;
+  cdiff::KappaLaw kap{};
+  double L = 0, spacing[3] = {0, 0, 0};
+  bool metrics_set = false;
+  double N_full = 0;
+
+  // FP64 state
+  double* u[2] = {nullptr, nullptr};
+  double* q[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  double* u_prev = nullptr;
+  double* Tdev = nullptr;
+  double* alpha = nullptr;
+  double* coef[9] = {nullptr};  // 2D: g11 g22 c12 ac bc; 3D: g11 g22 g33 cxe cxz cez ac bc gc
+  double *S = nullptr, *rho = nullptr, *tzA = nullptr, *tzB = nullptr;
+  double *S_store = nullptr, *rho_store = nullptr, *tzA_store = nullptr, *tzB_store = nullptr;
+  double* frozen[3] = {nullptr, nullptr, nullptr};  // alpha, theta, dtau (subcycle_conductivity = false)
+  double* part_sum = nullptr;
+  double* part_max = nullptr;
+  int nparts = 0;
+  double* red = nullptr;
+  double* red_h = nullptr;
+  cdiff::Ctrl* ctrl = nullptr;
+  cdiff::Ctrl* ctrl_h = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<void*> owned;
+
+  // FP32 shadows (defect-correction mode)
+  float* u32[2] = {nullptr, nullptr};
+  float* q32[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  float* base32 = nullptr;  // u_prev / Delta-t + S + correction, per cell
+  float* coef32[9] = {nullptr};
+  float *S32 = nullptr, *rho32 = nullptr, *tzA32 = nullptr, *tzB32 = nullptr;
+  float* frozen32[3] = {nullptr, nullptr, nullptr};
+  std::vector<void*> owned32;
+  bool coefs32_valid = false, aux32_valid = false, q32_valid = false;
+
+  // sweep decomposition
+  int strips = 0, chunks = 0, ch = 0, ntiles = 0, nblocks = 0;
+  int strips3[2] = {0, 0};
+
+  // persistent solver state
+  int cur = 0;        // ping-pong index of the current q (and u) buffers
+  int last_iter = 0;  // solver.last_err_check
+  bool have_last = false;
+  int last_src = 0;  // buffer holding the pre-update iterate of the last iteration
+  double last_dt = 0;
+  double last_Vpdtau = 0;
+  int last_subcycle = 1;
+  int enforce_pos_last = 0;
+  bool t_written = false;
+
+  std::string err;
+
+  int fail(int code, const std::string& msg);
+  int hip_fail(hipError_t e, const char* what);
+  void configure_sweep();
+  int step(double dt, const cdiff_opts& o, cdiff_stats* stats, double* next_dt);
+  int run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int first_iter_base);
+  int enqueue_next_dt(const double* unew, const double* uold, double dt, double tol, double muc, double growth);
+  int get_field(int field, double* out);
+  int validation_message(int status);
+  int step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cdiff_stats* stats);
+  int launch3d_fp64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check);
+};

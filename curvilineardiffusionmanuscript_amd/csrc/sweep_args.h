@@ -1,0 +1,42 @@
+// sweep_args.h -- kernel argument blocks of the fused APT sweeps.
+#pragma once
+#include "cdiff_dev.h"
+
+namespace cdiff {
+
+template <typename T>
+struct Sweep2DArgs {
+  Geo g;
+  Ghost gh;
+  KappaLaw kap;
+  IterConst ic;
+  int harmonic;
+  int ntiles, chunks, ch;
+  const T* __restrict__ u_src;
+  T* __restrict__ u_dst;
+  const T* __restrict__ qx_src;
+  T* __restrict__ qx_dst;
+  const T* __restrict__ qy_src;
+  T* __restrict__ qy_dst;
+  const T* __restrict__ u_prev;
+  const T* __restrict__ g11;
+  const T* __restrict__ g22;
+  const T* __restrict__ c12;
+  const T* __restrict__ ac;
+  const T* __restrict__ bc;
+  const T* __restrict__ S;    // nullable: source term
+  const T* __restrict__ rho;  // nullable: density (1 when null)
+  const T* __restrict__ tzA;  // nullable: forcing f = tzA + tzB * alpha
+  const T* __restrict__ tzB;
+  const T* __restrict__ frozen_alpha;  // nullable: subcycle_conductivity = false
+  const T* __restrict__ frozen_theta;
+  const T* __restrict__ frozen_dtau;
+  double* part_sum;
+  double* part_max;
+  const Ctrl* ctrl;
+};
+
+template <typename T>
+void launch_sweep2d(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s);
+
+}  // namespace cdiff

@@ -1,0 +1,221 @@
+/*
+ * cdiff.h -- C ABI of libcdiff, the MI355X (gfx950) implementation of
+ * CurvilinearDiffusion.jl's accelerated pseudo-transient (APT) solver.
+ *
+ * The reference's plug-in point is Julia dispatch on the backend type of
+ * PseudoTransientSolver{N,T,BE} (src/pseudo_transient/PseudoTransientSolver.jl:28,44,
+ * backend chosen by `backend=` at :49).  A backend marker type whose methods
+ * `ccall` these entry points replaces, one for one:
+ *
+ *   cdiff_create / cdiff_set_metrics   PseudoTransientSolver(mesh, bcs; backend, mean, T)
+ *                                      (PseudoTransientSolver.jl:48-103, phys_dims :122-153,
+ *                                      get_metric_cache :105-120)
+ *   cdiff_set_density, cdiff_set_source  the rho argument and solver.source_term
+ *                                      (PseudoTransientSolver.jl:31,184; drivers copy! into
+ *                                      source_term, e.g. test/integrated/2d/gaussian/run.jl:166)
+ *   cdiff_update_conductivity          update_conductivity!(solver, mesh, T, rho, cp, kappa)
+ *                                      (src/conductivity.jl:15-30,76-84)
+ *   cdiff_step                         nonlinear_thermal_conduction_step!(solver, mesh, T, rho,
+ *                                      cp, kappa, dt; kwargs...) -> (stats, next_dt)
+ *                                      (src/nonlinear_thermal_conduction.jl:51-72) which runs
+ *                                      PseudoTransientScheme.step! (PseudoTransientSolver.jl:180-351)
+ *   cdiff_apply_bcs                    applybcs!(bcs, mesh, u, nhalo)
+ *                                      (src/boundary_conditions/boundary_operators.jl:26-30)
+ *   cdiff_next_dt                      next_dt(u_new, u_prev, dt; tol, maximum_u_change,
+ *                                      timestep_growth_factor) (src/timestep_control.jl:33-54)
+ *   cdiff_get_field                    reads of solver.u, .u_prev, .res, .q, .q', .alpha,
+ *                                      .dtau_rho, .theta_r_dtau, .source_term, .cache
+ *                                      (PseudoTransientSolver.jl:28-46, to_vtk :380-412)
+ *
+ * Conventions.
+ *  - Every array is FP64, column-major (i fastest) and full-size: the mesh
+ *    halo is included, exactly as the Julia arrays.  Host arrays are borrowed
+ *    for the duration of a call.
+ *  - Device memory, the device-resident temperature and the persistent solver
+ *    state (fluxes q, the iteration count of the previous step, alpha) belong
+ *    to the handle.  A handle is bound to the device current at creation and
+ *    is not re-entrant; calls on one handle must not overlap.
+ *  - Every call returns a cdiff_status; cdiff_last_error(h) gives the message,
+ *    worded as the reference's error() text where one exists.
+ */
+#ifndef CDIFF_H
+#define CDIFF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cdiff_solver cdiff_solver;
+
+typedef enum {
+  CDIFF_OK = 0,
+  CDIFF_NONFINITE = 1,      /* "Non-finite error detected! ..."   PseudoTransientSolver.jl:302-307 */
+  CDIFF_MAX_ITER = 2,       /* "Maximum iteration limit reached"  PseudoTransientSolver.jl:309-314 */
+  CDIFF_INVALID_ALPHA = 3,  /* validate_scalar(alpha, ...)        validity_checks.jl:1-42 */
+  CDIFF_INVALID_SOURCE = 4, /* validate_scalar(source_term, ...)  validity_checks.jl:1-42 */
+  CDIFF_BAD_ARG = 5,        /* argument / @assert failures         PseudoTransientSolver.jl:225-227 */
+  CDIFF_HIP_ERROR = 6,      /* a HIP runtime call failed */
+  CDIFF_NOT_READY = 7       /* metrics not set */
+} cdiff_status;
+
+/* Boundary condition kinds (boundary_operators.jl:11-17). */
+typedef enum {
+  CDIFF_BC_NEUMANN = 0,
+  CDIFF_BC_DIRICHLET = 1,
+  CDIFF_BC_PERIODIC = 2,
+  CDIFF_BC_FIXED_NEG_SYMMETRY = 3
+} cdiff_bc_kind;
+
+typedef struct {
+  int32_t kind;  /* cdiff_bc_kind */
+  int32_t pad;
+  double value;  /* DirichletBC value (bc.val.u in 2D, bc.val in 3D) */
+} cdiff_bc;
+
+/* Face means (src/averaging.jl:7-8); `mean=` of the constructor. */
+typedef enum { CDIFF_MEAN_HARMONIC = 0, CDIFF_MEAN_ARITHMETIC = 1 } cdiff_mean;
+
+/* Conductivity law kappa(rho, T) = kappa0 * |rho|^rho_exp * |T|^t_exp, and 0
+ * when T is not finite.  Covers every closure the reference drivers pass
+ * (constant, 2.5|T|^3, T^3, T^(5/2)); Julia closures cannot cross ccall.
+ * Integer exponents 0..3 are evaluated by repeated multiplication (Julia
+ * literal powers), others with pow(). */
+typedef struct {
+  double kappa0;
+  double t_exp;
+  double rho_exp;
+} cdiff_kappa;
+
+/* Precision of the pseudo-iterations. */
+typedef enum {
+  CDIFF_PREC_FP64 = 0,   /* the reference's arithmetic, FP64 throughout */
+  CDIFF_PREC_FP32_DC = 1 /* FP32 APT sweeps inside an FP64 defect-correction loop */
+} cdiff_precision;
+
+/* step! keyword arguments (PseudoTransientSolver.jl:188-202) and next_dt
+ * keywords (timestep_control.jl:33-42).  cdiff_default_opts fills the
+ * reference defaults. */
+typedef struct {
+  int32_t max_iter;              /* 1500 */
+  int32_t norm_kind;             /* 0: sqrt(sum r^2 / N) (CPU L2_norm, residuals.jl:8-22);
+                                    1: norm(r, Inf)/sqrt(N) (GPU L2_norm, residuals.jl:1-6) */
+  double rtol;                   /* 1e-5 */
+  double atol;                   /* sqrt(eps(Float64)) */
+  double cfl;                    /* 1/sqrt(3) */
+  int32_t apply_cutoff;          /* false */
+  int32_t calculate_next_dt;     /* true */
+  int32_t subcycle_conductivity; /* true */
+  int32_t enforce_positivity;    /* false */
+  double tol;                    /* 1e-3 */
+  double maximum_u_change;       /* 0.05 */
+  double timestep_growth_factor; /* 1.1 */
+  int32_t precision;             /* cdiff_precision, CDIFF_PREC_FP64 */
+  int32_t inner_iters;           /* FP32_DC: FP32 sweeps per FP64 correction (0 = auto) */
+} cdiff_opts;
+
+/* stats = (rel_err, abs_err, niter, time) of step! (PseudoTransientSolver.jl:349). */
+typedef struct {
+  double rel_err;
+  double abs_err;
+  int32_t niter;   /* pseudo-iterations of this step */
+  int32_t nchecks; /* residual evaluations */
+  double time;     /* seconds, device-synchronised wall time of the step */
+  int32_t outer;   /* FP32_DC: FP64 defect-correction rounds */
+  int32_t pad;
+} cdiff_stats;
+
+/* Fields for cdiff_get_field / cdiff_set_field. */
+typedef enum {
+  CDIFF_FIELD_U = 0,       /* solver.u (device temperature) */
+  CDIFF_FIELD_U_PREV = 1,  /* solver.u_prev */
+  CDIFF_FIELD_RES = 2,     /* solver.res of the last check */
+  CDIFF_FIELD_ALPHA = 3,   /* solver.alpha */
+  CDIFF_FIELD_DTAU_RHO = 4,
+  CDIFF_FIELD_THETA_R_DTAU = 5,
+  CDIFF_FIELD_SOURCE = 6,
+  CDIFF_FIELD_Q = 8,       /* + axis: solver.q.{x,y,z} (damped flux) */
+  CDIFF_FIELD_QPRIME = 12, /* + axis: solver.q'.{x,y,z} (plain flux) */
+  CDIFF_FIELD_CACHE = 16,  /* + axis: solver.cache.{alpha,beta,gamma} */
+  CDIFF_FIELD_DENSITY = 20
+} cdiff_field;
+
+void cdiff_default_opts(cdiff_opts *opts);
+
+/* Create a solver for a mesh of `ndim` (2 or 3) with full (halo-inclusive)
+ * dimensions dims_full[0..ndim-1] and mesh halo `nhalo` (>= 1).  bcs[2*ndim] in
+ * (ilo, ihi, jlo, jhi, klo, khi) order -- the NamedTuple position of the
+ * reference (boundary_operators.jl:19-28).  Density defaults to 1, the source
+ * to 0. */
+int cdiff_create(int ndim, const int64_t *dims_full, int nhalo, int mean, const cdiff_bc *bcs,
+                 const cdiff_kappa *kappa, double cp, cdiff_solver **out);
+
+/* Mesh metrics (all full-size FP64 arrays):
+ *   cell[a*ndim + c]             = d(xi_a)/d(x_c) at cell centres (xi.x1, xi.x2, eta.x1, ...)
+ *   edge[b*(ndim*ndim+1) + a*ndim + c] = conservative metric xi_hat_a . e_c on the
+ *                                  b+1/2 faces; edge[b*(ndim*ndim+1) + ndim*ndim] = J there
+ *   L, spacing[ndim]             = phys_dims (PseudoTransientSolver.jl:122-153)
+ * The metric cache alpha_c/beta_c/gamma_c (mesh_metric_cache.jl:39-250) and
+ * the divergence coefficients are computed here once, in FP64. */
+int cdiff_set_metrics(cdiff_solver *h, const double *const *cell, const double *const *edge, double L,
+                      const double *spacing);
+
+/* The conductivity law and heat capacity of the following steps: the kappa and
+ * c_p arguments of nonlinear_thermal_conduction_step! and update_conductivity!
+ * (src/nonlinear_thermal_conduction.jl:51-72, src/conductivity.jl:15). */
+int cdiff_set_kappa(cdiff_solver *h, const cdiff_kappa *kappa, double cp);
+
+int cdiff_set_density(cdiff_solver *h, const double *rho);   /* NULL resets to 1 */
+int cdiff_set_source(cdiff_solver *h, const double *source); /* NULL resets to 0 */
+
+/* Twilight-zone forcing f = A + B * alpha added to the update and the residual
+ * (update.jl:23-71, residuals.jl:49-99): B carries the alpha dependence of
+ * define_forcing_wavy (twilight_zone.jl:175-191).  NULL, NULL removes it. */
+int cdiff_set_forcing(cdiff_solver *h, const double *A, const double *B);
+
+/* update_conductivity!(solver, mesh, T, rho, cp, kappa): alpha = |kappa/(rho cp)|
+ * over the full array (src/conductivity.jl:15-30,76-84). */
+int cdiff_update_conductivity(cdiff_solver *h, const double *T);
+
+/* One time step.  T (full, host) is read, and overwritten with solver.u at the
+ * end, like copy!(T, solver.u) (PseudoTransientSolver.jl:229-230,346). */
+int cdiff_step(cdiff_solver *h, double *T, double dt, const cdiff_opts *opts, cdiff_stats *stats,
+               double *next_dt);
+
+/* Device-resident variants: the temperature stays in HBM between steps.
+ * cdiff_set_temperature / cdiff_get_temperature move it; cdiff_step_resident
+ * is cdiff_step without the two host copies. */
+int cdiff_set_temperature(cdiff_solver *h, const double *T);
+int cdiff_get_temperature(cdiff_solver *h, double *T);
+int cdiff_step_resident(cdiff_solver *h, double dt, const cdiff_opts *opts, cdiff_stats *stats,
+                        double *next_dt);
+
+/* applybcs!(bcs, mesh, u, 1) on a host array (uploaded, filled, downloaded). */
+int cdiff_apply_bcs(cdiff_solver *h, double *u);
+
+/* next_dt(u_new, u_prev, dt; ...) on host arrays of the solver's full size. */
+int cdiff_next_dt(cdiff_solver *h, const double *u_new, const double *u_prev, double dt, double tol,
+                  double maximum_u_change, double timestep_growth_factor, double *out);
+
+/* Copy a solver field (cdiff_field) to a host array of the full size. */
+int cdiff_get_field(cdiff_solver *h, int field, double *out);
+
+/* Overwrite the persistent state: field CDIFF_FIELD_Q+axis or CDIFF_FIELD_ALPHA. */
+int cdiff_set_field(cdiff_solver *h, int field, const double *in);
+
+/* The error-check state solver.last_err_check (PseudoTransientSolver.jl:45,83,321). */
+int cdiff_get_last_iter(cdiff_solver *h, int32_t *out);
+int cdiff_set_last_iter(cdiff_solver *h, int32_t v);
+
+/* Zero q and q', set last_err_check = 0 (a freshly constructed solver). */
+int cdiff_reset_state(cdiff_solver *h);
+
+const char *cdiff_last_error(const cdiff_solver *h);
+const char *cdiff_status_string(int status);
+int cdiff_destroy(cdiff_solver *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

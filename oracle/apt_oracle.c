@@ -1,0 +1,591 @@
+/*
+ * apt_oracle.c -- CPU restatement of CurvilinearDiffusion.jl's accelerated
+ * pseudo-transient (APT) solver, FP64, unfused, in the reference's pass order.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * curvilineardiffusionmanuscript_amd/csrc: it is loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and never by the
+ * product path.
+ *
+ * Every routine cites the reference function it restates (paths relative to
+ * the reference repository root).  Julia semantics that matter for parity are
+ * reproduced explicitly:
+ *   - Bool "strong zero":  x * isfinite(x)  ==  isfinite(x) ? x : copysign(0,x)
+ *   - left-associative n-ary `+`, literal powers x^2 -> x*x, x^3 -> x*x*x
+ *   - round() ties-to-even for the error-check schedule
+ *   - isapprox(x, y; rtol) with atol = 0
+ * Arrays are column-major (i fastest) with the mesh halo included, exactly as
+ * the Julia arrays are laid out.  Indices here are 0-based.
+ *
+ * Parity status of each function is listed in the oracle header table in
+ * oracle/README.md.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "apt_oracle.h"
+
+#define PI_D 3.141592653589793
+
+static inline double strong_zero(double x) { return isfinite(x) ? x : copysign(0.0, x); }
+
+/* src/averaging.jl:7-8 */
+static inline double mean2(int kind, double a, double b) {
+  if (kind == ORA_MEAN_HARMONIC) return (2.0 * a * b) / (a + b);
+  return (a + b) / 2.0;
+}
+
+static inline int64_t lin(const ora_solver *s, int64_t i, int64_t j, int64_t k) {
+  return i + s->n[0] * (j + s->n[1] * k);
+}
+
+static inline int64_t stride_of(const ora_solver *s, int axis) {
+  return axis == 0 ? 1 : (axis == 1 ? s->n[0] : s->n[0] * s->n[1]);
+}
+
+/* Julia x^p for the kappa law: integer literals 0..3 use literal_pow
+ * (repeated multiplication), everything else pow(). */
+static inline double jl_pow(double x, double p, int p_is_int) {
+  if (p_is_int) {
+    int ip = (int)p;
+    if (ip == 0) return 1.0;
+    if (ip == 1) return x;
+    if (ip == 2) return x * x;
+    if (ip == 3) return x * x * x;
+  }
+  return pow(x, p);
+}
+
+/* The closed conductivity law that crosses the C boundary:
+ * kappa(rho, T) = kappa0 * |rho|^rho_exp * |T|^t_exp, and 0 when T is not
+ * finite (the guard every driver closure has, e.g.
+ * test/integrated/2d/gaussian/run.jl:129-135). */
+double ora_kappa(const ora_solver *s, double rho, double T) {
+  if (!isfinite(T)) return 0.0;
+  double rt = (s->rho_exp == 0.0) ? 1.0 : jl_pow(fabs(rho), s->rho_exp, s->rho_exp_is_int);
+  double tt = jl_pow(fabs(T), s->t_exp, s->t_exp_is_int);
+  if (s->rho_exp == 0.0) return s->kappa0 * tt;
+  return s->kappa0 * rt * tt;
+}
+
+/* ------------------------------------------------------------------------- */
+
+int ora_init(ora_solver *s) {
+  if (s->ndim != 2 && s->ndim != 3) return ORA_BAD_ARG;
+  if (s->ndim == 2) s->n[2] = 1;
+  s->N = s->n[0] * s->n[1] * s->n[2];
+  for (int a = 0; a < 3; a++) {
+    if (a < s->ndim) {
+      s->lo[a] = s->h;
+      s->hi[a] = s->n[a] - s->h - 1;
+    } else {
+      s->lo[a] = 0;
+      s->hi[a] = 0;
+    }
+  }
+  s->err[0] = 0;
+  return ORA_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Boundary conditions: src/boundary_conditions/boundary_operators.jl:26-263.
+ * Location is the position in the bcs tuple (ilo, ihi, jlo, jhi, klo, khi,
+ * :19-24).  Strips span the domain range of the other axes (corners are never
+ * written). */
+
+static void strip_loop(const ora_solver *s, int axis, int64_t plane, int64_t src_plane, int mode,
+                       double val) {
+  /* mode 0: copy src plane; 1: set val; 2: set -src */
+  int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+  if (s->ndim == 2) {
+    a1 = 1 - axis;
+    a2 = 2;
+  }
+  for (int64_t y = s->lo[a2]; y <= s->hi[a2]; y++) {
+    for (int64_t x = s->lo[a1]; x <= s->hi[a1]; x++) {
+      int64_t c[3], d[3];
+      c[axis] = plane;
+      c[a1] = x;
+      c[a2] = y;
+      d[axis] = src_plane;
+      d[a1] = x;
+      d[a2] = y;
+      int64_t dst = lin(s, c[0], c[1], c[2]);
+      int64_t src = lin(s, d[0], d[1], d[2]);
+      if (mode == 0)
+        s->tmp_u[dst] = s->tmp_u[src];
+      else if (mode == 1)
+        s->tmp_u[dst] = val;
+      else
+        s->tmp_u[dst] = -s->tmp_u[src];
+    }
+  }
+}
+
+int ora_applybcs(ora_solver *s, double *u, int nhalo) {
+  s->tmp_u = u;
+  int nb = 2 * s->ndim;
+  for (int loc = 0; loc < nb; loc++) {
+    int axis = loc / 2, hi = loc % 2;
+    const ora_bc *bc = &s->bcs[loc];
+    int64_t lo_a = s->lo[axis], hi_a = s->hi[axis];
+    switch (bc->kind) {
+      case ORA_BC_NEUMANN: /* :55-119, nhalo layers copy the edge */
+        for (int nh = 1; nh <= nhalo; nh++) {
+          if (!hi) strip_loop(s, axis, lo_a - nh, lo_a, 0, 0.0);
+          else strip_loop(s, axis, hi_a + nh, hi_a, 0, 0.0);
+        }
+        break;
+      case ORA_BC_DIRICHLET:
+        if (s->ndim == 2) { /* :137-155, every halo layer to bc.val.u */
+          if (!hi)
+            for (int64_t p = 0; p < lo_a; p++) strip_loop(s, axis, p, p, 1, bc->value);
+          else
+            for (int64_t p = hi_a + 1; p < s->n[axis]; p++) strip_loop(s, axis, p, p, 1, bc->value);
+        } else { /* :178-200, one layer to bc.val */
+          if (!hi) strip_loop(s, axis, lo_a - 1, lo_a - 1, 1, bc->value);
+          else strip_loop(s, axis, hi_a + 1, hi_a + 1, 1, bc->value);
+        }
+        break;
+      case ORA_BC_PERIODIC: /* :220-263, acts at the lo location only */
+        if (!hi) {
+          strip_loop(s, axis, lo_a - 1, hi_a, 0, 0.0);
+          strip_loop(s, axis, hi_a + 1, lo_a, 0, 0.0);
+        }
+        break;
+      case ORA_BC_FIXED_NEG_SYMMETRY: /* :157-175 (2D) */
+        if (s->ndim != 2) return ORA_BAD_ARG;
+        if (!hi) strip_loop(s, axis, lo_a - 1, lo_a, 2, 0.0);
+        else strip_loop(s, axis, hi_a + 1, hi_a, 2, 0.0);
+        break;
+      default:
+        return ORA_BAD_ARG;
+    }
+  }
+  return ORA_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* src/pseudo_transient/conductivity.jl:1-13 (CPU path, authoritative) and the
+ * exported src/conductivity.jl:15-30,76-84: alpha = |kappa(rho,T)/(rho*cp)|
+ * over the full array including halos and corners. */
+void ora_update_conductivity(ora_solver *s, const double *T) {
+  const double cp = s->cp;
+#pragma omp parallel for schedule(static)
+  for (int64_t idx = 0; idx < s->N; idx++) {
+    double kappa = ora_kappa(s, s->rho[idx], T[idx]);
+    s->alpha[idx] = fabs(kappa / (s->rho[idx] * cp));
+  }
+}
+
+/* src/pseudo_transient/iteration_parameters.jl:2-15, over the full array. */
+void ora_update_iteration_params(ora_solver *s, double Vpdtau, double dt) {
+  const double L = s->L;
+  const double pi2 = PI_D * PI_D; /* Julia pi^2 == Float64(pi)^2 */
+#pragma omp parallel for schedule(static)
+  for (int64_t idx = 0; idx < s->N; idx++) {
+    double a = s->alpha[idx];
+    double Re = PI_D + sqrt(pi2 + (L * L * s->rho[idx]) / (a * dt));
+    double dtr = (Vpdtau * L / (a * Re));
+    double thr = (L / (Vpdtau * Re));
+    s->dtau[idx] = strong_zero(dtr);
+    s->theta[idx] = strong_zero(thr);
+  }
+}
+
+/* src/pseudo_transient/flux/fluxes.jl:5-26 and fluxes_cpu.jl:2-115: faces in
+ * expand_lower(domain, axis) only; all other faces are never written. */
+void ora_compute_flux(ora_solver *s) {
+  for (int axis = 0; axis < s->ndim; axis++) {
+    int64_t st = stride_of(s, axis);
+    double *q = s->q[axis], *qp = s->qp[axis];
+    int64_t lo[3] = {s->lo[0], s->lo[1], s->lo[2]};
+    lo[axis] -= 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t k = lo[2]; k <= s->hi[2]; k++)
+      for (int64_t j = lo[1]; j <= s->hi[1]; j++)
+        for (int64_t i = lo[0]; i <= s->hi[0]; i++) {
+          int64_t id = lin(s, i, j, k), ip = id + st;
+          double up = s->u[ip], uc = s->u[id];
+          double qc = q[id];
+          double ap = s->alpha[ip], ac = s->alpha[id];
+          double tp = s->theta[ip], tc = s->theta[id];
+          double abar = mean2(s->mean, ac, ap);
+          double tbar = mean2(s->mean, tc, tp);
+          double du = up - uc;
+          double qplain = -abar * du;
+          q[id] = (qc * tbar + qplain) / (1.0 + tbar);
+          qp[id] = -abar * du; /* fluxprime_kernel! recomputes the mean */
+        }
+  }
+}
+
+/* src/pseudo_transient/flux_divergence.jl:7-42 (2D) and :119-239 (3D). */
+double ora_flux_divergence(const ora_solver *s, double *const *q, int64_t i, int64_t j, int64_t k) {
+  const double *qi = q[0], *qj = q[1];
+  if (s->ndim == 2) {
+    int64_t c = lin(s, i, j, 0);
+    int64_t NI = s->n[0];
+    double xix = s->cell[0][c], xiy = s->cell[1][c];
+    double etx = s->cell[2][c], ety = s->cell[3][c];
+    double dqi = qi[c] - qi[c - 1];
+    double dqj = qj[c] - qj[c - NI];
+    double t1 = (xix * xix + xiy * xiy) * dqi;
+    double t2 = (etx * etx + ety * ety) * dqj;
+    double t3 = 0.25 * (etx * xix + ety * xiy) *
+                ((qi[c + NI] + qi[c - 1 + NI]) - (qi[c - NI] + qi[c - 1 - NI]));
+    double t4 = 0.25 * (etx * xix + ety * xiy) *
+                ((qj[c + 1] + qj[c + 1 - NI]) - (qj[c - 1] + qj[c - 1 - NI]));
+    double t5 = s->cache[0][c] * (0.5 * (qi[c] + qi[c - 1]));
+    double t6 = s->cache[1][c] * (0.5 * (qj[c] + qj[c - NI]));
+    return t1 + t2 + t3 + t4 + t5 + t6;
+  } else {
+    const double *qk = q[2];
+    int64_t c = lin(s, i, j, k);
+    int64_t sI = 1, sJ = s->n[0], sK = s->n[0] * s->n[1];
+    double xix = s->cell[0][c], xiy = s->cell[1][c], xiz = s->cell[2][c];
+    double etx = s->cell[3][c], ety = s->cell[4][c], etz = s->cell[5][c];
+    double zex = s->cell[6][c], zey = s->cell[7][c], zez = s->cell[8][c];
+    double dqi = qi[c] - qi[c - sI];
+    double dqj = qj[c] - qj[c - sJ];
+    double dqk = qk[c] - qk[c - sK];
+    double a1 = (xix * xix + xiy * xiy + xiz * xiz) * dqi;
+    double a2 = (etx * etx + ety * ety + etz * etz) * dqj;
+    double a3 = (zex * zex + zey * zey + zez * zez) * dqk;
+    double gxe = etx * xix + ety * xiy + etz * xiz;
+    double gez = zex * etx + zey * ety + zez * etz;
+    double gxz = zex * xix + zey * xiy + zez * xiz;
+    double qi_eta = 0.5 * gxe * (0.5 * (qi[c + sJ] + qi[c - sI + sJ]) - 0.5 * (qi[c - sJ] + qi[c - sI - sJ]));
+    double qj_xi = 0.5 * gxe * (0.5 * (qj[c + sI] + qj[c + sI - sJ]) - 0.5 * (qj[c - sI] + qj[c - sI - sJ]));
+    double qj_zeta = 0.5 * gez * (0.5 * (qj[c + sK] + qj[c - sJ + sK]) - 0.5 * (qj[c - sK] + qj[c - sJ - sK]));
+    double qk_eta = 0.5 * gez * (0.5 * (qk[c + sJ] + qk[c + sJ - sK]) - 0.5 * (qk[c - sJ] + qk[c - sJ - sK]));
+    double qi_zeta = 0.5 * gxz * (0.5 * (qi[c + sK] + qi[c - sI + sK]) - 0.5 * (qi[c - sK] + qi[c - sI - sK]));
+    double qk_xi = 0.5 * gxz * (0.5 * (qk[c + sI] + qk[c + sI - sK]) - 0.5 * (qk[c - sI] + qk[c - sI - sK]));
+    double m1 = s->cache[0][c] * (0.5 * (qi[c] + qi[c - sI]));
+    double m2 = s->cache[1][c] * (0.5 * (qj[c] + qj[c - sJ]));
+    double m3 = s->cache[2][c] * (0.5 * (qk[c] + qk[c - sK]));
+    return a1 + a2 + a3 + qi_eta + qi_zeta + qj_xi + qj_zeta + qk_eta + qk_xi + m1 + m2 + m3;
+  }
+}
+
+/* src/pseudo_transient/update.jl:2-21,73-119 (u updated in place over the
+ * domain; the divergence reads only q so the in-place write is safe). */
+void ora_compute_update(ora_solver *s, double dt) {
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int64_t k = s->lo[2]; k <= s->hi[2]; k++)
+    for (int64_t j = s->lo[1]; j <= s->hi[1]; j++)
+      for (int64_t i = s->lo[0]; i <= s->hi[0]; i++) {
+        int64_t c = lin(s, i, j, k);
+        double dq = ora_flux_divergence(s, s->q, i, j, k);
+        double dtr = s->dtau[c];
+        double rhs = s->u_prev[c] / dt - dq + s->S[c];
+        /* update.jl:67-70: the twilight-zone forcing f is added last */
+        if (s->tzA) rhs = rhs + (s->tzA[c] + s->tzB[c] * s->alpha[c]);
+        s->u[c] = ((s->u[c] + dtr * rhs) / (1.0 + dtr / dt));
+      }
+}
+
+/* Julia isapprox(x, y; rtol, atol=0) */
+static inline int jl_isapprox(double x, double y, double rtol) {
+  if (x == y) return 1;
+  if (!(isfinite(x) && isfinite(y))) return 0;
+  double m = fmax(fabs(x), fabs(y));
+  return fabs(x - y) <= rtol * m;
+}
+
+/* src/pseudo_transient/residuals.jl:24-47,105-160: residual with the plain
+ * flux q' over the domain; halo entries of res are never written. */
+void ora_update_residual(ora_solver *s, double dt) {
+  const double eps = 2.220446049250313e-16;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int64_t k = s->lo[2]; k <= s->hi[2]; k++)
+    for (int64_t j = s->lo[1]; j <= s->hi[1]; j++)
+      for (int64_t i = s->lo[0]; i <= s->hi[0]; i++) {
+        int64_t c = lin(s, i, j, k);
+        double dq = ora_flux_divergence(s, s->qp, i, j, k);
+        double un = s->u[c], um = s->u_prev[c];
+        double du = un - um;
+        if (jl_isapprox(un, um, eps)) du = copysign(0.0, du);
+        double r = -du / dt - dq + s->S[c];
+        if (s->tzA) r = r + (s->tzA[c] + s->tzB[c] * s->alpha[c]); /* residuals.jl:217 */
+        s->res[c] = r;
+      }
+}
+
+/* src/pseudo_transient/residuals.jl:1-22.  kind 0: CPU RMS sqrt(sum a^2 / N);
+ * kind 1: GPU norm(a, Inf)/sqrt(N).  N is the full length (halo included).
+ * The sum is taken in a fixed order (per-line partials, then sequential) so
+ * the result does not depend on the thread count. */
+double ora_l2_norm(const ora_solver *s, const double *a, int kind) {
+  int64_t NI = s->n[0], lines = s->n[1] * s->n[2];
+  if (kind == 1) {
+    double m = 0.0;
+    for (int64_t idx = 0; idx < s->N; idx++) {
+      double v = fabs(a[idx]);
+      if (isnan(v)) return NAN;
+      if (v > m) m = v;
+    }
+    return m / sqrt((double)s->N);
+  }
+  double *part = (double *)malloc(sizeof(double) * (size_t)lines);
+#pragma omp parallel for schedule(static)
+  for (int64_t l = 0; l < lines; l++) {
+    double acc = 0.0;
+    const double *row = a + l * NI;
+    for (int64_t i = 0; i < NI; i++) acc += row[i] * row[i];
+    part[l] = acc;
+  }
+  double tot = 0.0;
+  for (int64_t l = 0; l < lines; l++) tot += part[l];
+  free(part);
+  return sqrt(tot / (double)s->N);
+}
+
+/* src/pseudo_transient/PseudoTransientSolver.jl:357-364 */
+void ora_cutoff(ora_solver *s, double *A) {
+#pragma omp parallel for schedule(static)
+  for (int64_t idx = 0; idx < s->N; idx++) {
+    double a = A[idx];
+    double v = (0.5 * (fabs(a) + a));
+    A[idx] = isfinite(a) ? v : copysign(0.0, v);
+  }
+}
+
+/* src/timestep_control.jl:33-65 (CPU path): next_dt(u_new, u_prev, dt). */
+double ora_next_dt(const ora_solver *s, const double *unew, const double *uold, double dt, double tol,
+                   double maximum_u_change, double growth) {
+  double umax = -INFINITY;
+  for (int64_t idx = 0; idx < s->N; idx++) {
+    double v = uold[idx];
+    if (isnan(v)) {
+      umax = NAN;
+      break;
+    }
+    if (v > umax) umax = v;
+  }
+  double u0 = umax * tol;
+  double mx = -INFINITY;
+  for (int64_t idx = 0; idx < s->N; idx++) {
+    double d = fabs(unew[idx] - uold[idx]) / (unew[idx] + u0);
+    d = strong_zero(d);
+    /* Julia max(x, y) with neither NaN: larger, +0 over -0 */
+    if (d > mx || (d == mx && signbit(mx) && !signbit(d))) mx = d;
+  }
+  double dtn = dt * sqrt(fabs(maximum_u_change / mx));
+  double g = growth * dt;
+  if (isnan(dtn) || isnan(g)) return NAN;
+  return dtn < g ? dtn : g;
+}
+
+/* src/validity_checks.jl:1-42 over the domain plus the one-layer halo strips
+ * of expand(domain, nhalo) (full extent of the other axes). Returns 0 when
+ * valid; otherwise writes the reference's message into s->err. */
+static int bad_value(double v, int positivity) {
+  if (!isfinite(v)) return 1;
+  if (positivity && !(v >= 0)) return 1;
+  return 0;
+}
+
+int ora_validate_scalar(ora_solver *s, const double *a, const char *name, int positivity) {
+  int bad = 0;
+  double mn = INFINITY, mx = -INFINITY;
+  int anynan = 0;
+  for (int64_t k = s->lo[2]; k <= s->hi[2]; k++)
+    for (int64_t j = s->lo[1]; j <= s->hi[1]; j++)
+      for (int64_t i = s->lo[0]; i <= s->hi[0]; i++) {
+        double v = a[lin(s, i, j, k)];
+        if (bad_value(v, positivity)) bad = 1;
+        if (isnan(v)) anynan = 1;
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+      }
+  if (bad) {
+    if (anynan) mn = mx = NAN;
+    snprintf(s->err, sizeof s->err, "Invalid %s in the domain extrema -> (%g, %g)", name, mn, mx);
+    return ORA_INVALID_SCALAR;
+  }
+  int64_t flo[3], fhi[3];
+  for (int a2 = 0; a2 < 3; a2++) {
+    flo[a2] = (a2 < s->ndim) ? s->lo[a2] - 1 : 0;
+    fhi[a2] = (a2 < s->ndim) ? s->hi[a2] + 1 : 0;
+  }
+  for (int axis = 0; axis < s->ndim; axis++) {
+    for (int side = 0; side < 2; side++) {
+      int64_t plane = side == 0 ? flo[axis] : fhi[axis];
+      int b = 0;
+      for (int64_t k = flo[2]; k <= fhi[2]; k++)
+        for (int64_t j = flo[1]; j <= fhi[1]; j++)
+          for (int64_t i = flo[0]; i <= fhi[0]; i++) {
+            int64_t c[3] = {i, j, k};
+            if (c[axis] != plane) continue;
+            if (bad_value(a[lin(s, i, j, k)], positivity)) b = 1;
+          }
+      if (b) {
+        snprintf(s->err, sizeof s->err, "Invalid %s in the %s halo region for axis: %d", name,
+                 side == 0 ? "lo" : "hi", axis + 1);
+        return ORA_INVALID_SCALAR;
+      }
+    }
+  }
+  return ORA_OK;
+}
+
+/* src/pseudo_transient/mesh_metric_cache.jl:39-102 (2D) and :104-250 (3D).
+ * cache_a = sum_b sum_c m_b,c * (E_b[a,c]/J_b |+1/2 - E_b[a,c]/J_b |-1/2),
+ * b = face axis (i, j, k), c = Cartesian component; m_b,c = d(xi_b)/d(x_c) at
+ * the cell centre.  The terms are added in the reference's order. */
+void ora_update_metric_cache(ora_solver *s) {
+  int nd = s->ndim;
+  int nm = nd * nd;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int64_t k = s->lo[2]; k <= s->hi[2]; k++)
+    for (int64_t j = s->lo[1]; j <= s->hi[1]; j++)
+      for (int64_t i = s->lo[0]; i <= s->hi[0]; i++) {
+        int64_t c = lin(s, i, j, k);
+        for (int a = 0; a < nd; a++) {
+          double acc = 0.0;
+          int first = 1;
+          for (int b = 0; b < nd; b++) {
+            int64_t cm = c - stride_of(s, b);
+            const double *Jb = s->edge[b * (nm + 1) + nm];
+            double Jp = Jb[c], Jm = Jb[cm];
+            for (int cc = 0; cc < nd; cc++) {
+              const double *E = s->edge[b * (nm + 1) + a * nd + cc];
+              double vp = E[c] / Jp;
+              double vm = E[cm] / Jm;
+              double term = s->cell[b * nd + cc][c] * (vp - vm);
+              if (first) {
+                acc = term;
+                first = 0;
+              } else
+                acc = acc + term;
+            }
+          }
+          s->cache[a][c] = acc;
+        }
+      }
+}
+
+/* PseudoTransientSolver.jl:206-208 (schedule) and :280 (check predicate). */
+void ora_schedule(int last_iter_count, int *min_err_check, int *err_interval) {
+  double m = nearbyint((double)last_iter_count * 0.9); /* ties-to-even */
+  int mi = (int)m;
+  int d = last_iter_count - mi; /* ceil of an Int is the Int */
+  int k = d / 2;
+  if (k < 1) k = 1;
+  *min_err_check = mi;
+  *err_interval = k;
+}
+
+int ora_is_check(int iter, int min_err_check, int err_interval) {
+  return (iter >= min_err_check && iter % err_interval == 0) || iter == 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* PseudoTransientScheme.step! (PseudoTransientSolver.jl:180-351). */
+int ora_step(ora_solver *s, double *T, double dt, const ora_opts *o, ora_stats *st, double *next_dt_out) {
+  int last = s->last_err_check;
+  int m, k;
+  ora_schedule(last, &m, &k);
+  const int nhalo = 1;
+  int iter = 0;
+  double rel = 2.0 * o->rtol, absr = 2.0 * o->atol, initL2 = INFINITY;
+  double smin = s->spacing[0];
+  for (int a = 1; a < s->ndim; a++)
+    if (s->spacing[a] < smin) smin = s->spacing[a];
+  double Vpdtau = o->cfl * smin;
+  if (!(dt > 0)) {
+    snprintf(s->err, sizeof s->err, "AssertionError: dt > 0");
+    return ORA_BAD_ARG;
+  }
+  for (int a = 0; a < s->ndim; a++)
+    if (!(s->spacing[a] > 0)) {
+      snprintf(s->err, sizeof s->err, "AssertionError: all(solver.spacing .> 0)");
+      return ORA_BAD_ARG;
+    }
+  if (!(Vpdtau > 0)) {
+    snprintf(s->err, sizeof s->err, "AssertionError: Vpdτ > 0");
+    return ORA_BAD_ARG;
+  }
+  memcpy(s->u, T, sizeof(double) * (size_t)s->N);
+  memcpy(s->u_prev, T, sizeof(double) * (size_t)s->N);
+  ora_update_metric_cache(s);
+  int rc = ora_validate_scalar(s, s->alpha, "diffusivity", o->enforce_positivity);
+  if (rc) return ORA_INVALID_ALPHA;
+  rc = ora_validate_scalar(s, s->S, "source_term", 0);
+  if (rc) return ORA_INVALID_SOURCE;
+
+  int nchecks = 0;
+  while (1) {
+    ora_applybcs(s, s->u, nhalo);
+    if (o->subcycle_conductivity || iter == 0) {
+      ora_update_conductivity(s, s->u);
+      ora_update_iteration_params(s, Vpdtau, dt);
+    }
+    iter += 1;
+    ora_compute_flux(s);
+    ora_compute_update(s, dt);
+    if (ora_is_check(iter, m, k)) {
+      ora_update_residual(s, dt);
+      nchecks++;
+      double L2 = ora_l2_norm(s, s->res, o->norm_kind);
+      if (iter == 1) initL2 = L2;
+      rel = L2 / initL2;
+      absr = L2;
+    }
+    if (!isfinite(rel) || !isfinite(absr)) {
+      st->rel_err = rel;
+      st->abs_err = absr;
+      st->niter = iter;
+      snprintf(s->err, sizeof s->err,
+               "Non-finite error detected! abs_error = %g, rel_error = %g, exiting...", absr, rel);
+      return ORA_NONFINITE;
+    }
+    if (iter > o->max_iter) {
+      st->rel_err = rel;
+      st->abs_err = absr;
+      st->niter = iter;
+      snprintf(s->err, sizeof s->err,
+               "Maximum iteration limit reached (%d), abs_error = %g, rel_error = %g, exiting...",
+               o->max_iter, absr, rel);
+      return ORA_MAX_ITER;
+    }
+    if (rel <= o->rtol || absr <= o->atol) break;
+  }
+  s->last_err_check = iter;
+  if (o->enforce_positivity)
+    for (int64_t idx = 0; idx < s->N; idx++) s->u[idx] = fabs(s->u[idx]);
+  if (o->apply_cutoff) ora_cutoff(s, s->u);
+  double ndt = dt;
+  if (o->calculate_next_dt)
+    ndt = ora_next_dt(s, s->u, s->u_prev, dt, o->tol, o->maximum_u_change, o->timestep_growth_factor);
+  memcpy(T, s->u, sizeof(double) * (size_t)s->N);
+  st->rel_err = rel;
+  st->abs_err = absr;
+  st->niter = iter;
+  st->nchecks = nchecks;
+  if (next_dt_out) *next_dt_out = ndt;
+  return ORA_OK;
+}
+
+int ora_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void ora_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}

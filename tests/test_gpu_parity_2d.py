@@ -1,0 +1,215 @@
+"""GPU parity of the 2D FP64 APT path (through the C ABI) against the oracle.
+
+Both sides get the same mesh metrics, initial temperature, BCs and
+conductivity law; after every time step the temperature, the pseudo-iteration
+count, the error estimates and next_dt are compared.  Tolerances: 1e-12
+relative max-norm on T (the north-star bar is 1e-10) and identical iteration
+counts.
+"""
+import numpy as np
+import pytest
+
+import apt_oracle as O
+from cases import gaussian_2d, hot_disk_2d
+from curvilineardiffusionmanuscript_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+BCMAP = {
+    "neumann": S.NeumannBC(),
+    "periodic": S.PeriodicBC(),
+    "neg": S.FixedNegSymmetryBC(),
+}
+
+
+def _bcs(spec):
+    out_dev, out_ora = [], []
+    for s in spec:
+        if isinstance(s, tuple):
+            out_dev.append(S.DirichletBC(s[1]))
+            out_ora.append(("dirichlet", s[1]))
+        else:
+            out_dev.append(BCMAP[s])
+            out_ora.append(({"neg": "fixed_neg_symmetry"}.get(s, s), 0.0))
+    return out_dev, out_ora
+
+
+def _run_pair(mesh, T0, bcs_spec, kappa, nsteps, dt0, *, rho=None, source=None, mean="harmonic", adaptive=False,
+              cp=1.0, **opts):
+    bd, bo = _bcs(bcs_spec)
+    law = S.PowerLawConductivity(*kappa)
+    dev = S.PseudoTransientSolver(mesh, bd, mean=mean)
+    ora = O.OracleSolver(mesh, bo, mean=mean, kappa=kappa, cp=cp, rho=rho, source=source)
+    if source is not None:
+        dev.source_term = source
+    rho_arr = np.ones(mesh.dims_full, order="F") if rho is None else np.asfortranarray(rho)
+    Td = np.array(T0, order="F", copy=True)
+    To = np.array(T0, order="F", copy=True)
+    S.update_conductivity(dev, mesh, Td, rho_arr, cp, law)
+    ora.update_conductivity(To)
+    dt_d = dt_o = dt0
+    out = []
+    for step in range(nsteps):
+        sd, nd = S.nonlinear_thermal_conduction_step(dev, mesh, Td, rho_arr, cp, law, dt_d, show_convergence=False,
+                                                     **opts)
+        so, no = ora.step(To, dt_o, **{k: v for k, v in _oracle_opts(opts).items()})
+        out.append((sd, nd, so, no, Td.copy(), To.copy()))
+        if adaptive:
+            dt_d, dt_o = nd, no
+    return dev, ora, out
+
+
+def _oracle_opts(opts):
+    m = dict(opts)
+    if "CFL" in m:
+        m["cfl"] = m.pop("CFL")
+    if "norm" in m:
+        m["norm_kind"] = 0 if m.pop("norm") == "cpu" else 1
+    return m
+
+
+def _assert_parity(out, rel=1e-12):
+    for k, (sd, nd, so, no, Td, To) in enumerate(out):
+        scale = np.max(np.abs(To))
+        err = np.max(np.abs(Td - To)) / scale
+        print(f"step {k}: niter dev={sd.niter} ora={so['niter']}  max rel err T={err:.3e}  "
+              f"abs_err dev={sd.abs_err:.6e} ora={so['abs_err']:.6e}  next_dt {nd!r} {no!r}")
+        assert sd.niter == so["niter"], f"step {k}: iteration counts differ"
+        assert err <= rel, f"step {k}: T differs by {err:.3e}"
+        assert abs(sd.abs_err - so["abs_err"]) <= 1e-9 * abs(so["abs_err"]) + 1e-300
+        assert abs(nd - no) <= 1e-12 * abs(no)
+
+
+def test_config1_gaussian_multi_strip():
+    # 70 x 45 cells: two strips of 62 columns (one ragged), several row chunks
+    mesh, T0 = gaussian_2d(45, ni=70)
+    _, _, out = _run_pair(mesh, T0, ["neumann"] * 4, (1.0, 0.0, 0.0), 4, 1e-4, rtol=0.0, atol=1e-8,
+                          max_iter=100000, calculate_next_dt=False)
+    _assert_parity(out, rel=0.0)  # kappa = 1: no pow(), the sweep is bit-exact
+
+
+def test_config2_hot_disk_nonlinear_dirichlet_adaptive():
+    mesh, T0 = hot_disk_2d(48, seed=1)
+    _, _, out = _run_pair(mesh, T0, [("dirichlet", 1.0), "neumann", "neumann", "neumann"], (1.0, 2.5, 0.0), 4, 1e-5,
+                          adaptive=True, rtol=0.0, atol=1e-8, max_iter=200000)
+    _assert_parity(out)
+
+
+@pytest.mark.parametrize(
+    "bcs",
+    [
+        ["periodic", "periodic", "neumann", ("dirichlet", 2.0)],
+        ["neg", "neumann", "periodic", "periodic"],
+        [("dirichlet", 0.5), ("dirichlet", 3.0), "neg", "neg"],
+        ["neumann", "periodic", "periodic", "neumann"],  # periodic only at ihi: ghost column keeps its value
+    ],
+)
+def test_boundary_condition_kinds(bcs):
+    mesh, T0 = gaussian_2d(40, ni=66)
+    _, _, out = _run_pair(mesh, T0, bcs, (2.5, 3.0, 0.0), 3, 1e-4, rtol=0.0, atol=1e-7, max_iter=100000)
+    _assert_parity(out, rel=0.0)  # integer exponent: literal power, bit-exact
+
+
+def test_frozen_conductivity_arithmetic_mean_source_density():
+    mesh, T0 = gaussian_2d(36, ni=64)
+    rng = np.random.default_rng(3)
+    rho = np.asfortranarray(1.0 + 0.5 * rng.random(mesh.dims_full))
+    src = np.zeros(mesh.dims_full, order="F")
+    h = mesh.nhalo
+    src[h:-h, h:-h] = 50.0 * rng.random((64, 36))
+    _, _, out = _run_pair(mesh, T0, ["neumann"] * 4, (1.5, 1.0, 1.0), 3, 2e-4, rho=rho, source=src,
+                          mean="arithmetic", cp=2.0, rtol=1e-9, atol=1e-9, max_iter=100000,
+                          subcycle_conductivity=False, enforce_positivity=True, apply_cutoff=True)
+    _assert_parity(out, rel=0.0)
+
+
+def test_gpu_norm_flavour_and_defaults():
+    mesh, T0 = gaussian_2d(32, ni=32)
+    _, _, out = _run_pair(mesh, T0, ["neumann"] * 4, (1.0, 0.0, 0.0), 3, 1e-4, norm="gpu")
+    _assert_parity(out, rel=0.0)
+
+
+def test_fields_after_step():
+    mesh, T0 = hot_disk_2d(40, seed=2)
+    dev, ora, out = _run_pair(mesh, T0, [("dirichlet", 1.0), "neumann", "neumann", "neumann"], (1.0, 3.0, 0.0), 2,
+                              1e-5, rtol=0.0, atol=1e-9, max_iter=100000)
+    _assert_parity(out, rel=0.0)
+    for a in range(2):
+        np.testing.assert_array_equal(dev.q[a], ora.q[a])
+        np.testing.assert_array_equal(dev.q_prime[a], ora.qp[a])
+        np.testing.assert_array_equal(dev.cache[a], ora.cache[a])
+    np.testing.assert_array_equal(dev.alpha, ora.alpha)
+    np.testing.assert_array_equal(dev.res, ora.res)
+    np.testing.assert_array_equal(dev.u_prev, ora.u_prev)
+    np.testing.assert_array_equal(dev.dtau_rho, ora.dtau)
+    np.testing.assert_array_equal(dev.theta_r_dtau, ora.theta)
+    assert dev.last_err_check == [ora.last_err_check]
+
+
+def test_max_iter_error_matches():
+    mesh, T0 = gaussian_2d(24)
+    law = S.PowerLawConductivity(1.0)
+    dev = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4)
+    ora = O.OracleSolver(mesh, [("neumann", 0.0)] * 4)
+    S.update_conductivity(dev, mesh, T0, 1.0, 1.0, law)
+    ora.update_conductivity(T0)
+    with pytest.raises(S.MaxIterError) as ed:
+        S.nonlinear_thermal_conduction_step(dev, mesh, T0.copy(order="F"), 1.0, 1.0, law, 1e-4, max_iter=5,
+                                            rtol=0.0, atol=1e-30, show_convergence=False)
+    with pytest.raises(O.OracleError) as eo:
+        ora.step(T0.copy(order="F"), 1e-4, max_iter=5, rtol=0.0, atol=1e-30)
+    assert str(ed.value).split(",")[0] == str(eo.value).split(",")[0] == "Maximum iteration limit reached (5)"
+
+
+def test_invalid_source_and_assertions():
+    mesh, T0 = gaussian_2d(20)
+    law = S.PowerLawConductivity(1.0)
+    dev = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4)
+    S.update_conductivity(dev, mesh, T0, 1.0, 1.0, law)
+    src = np.zeros(mesh.dims_full, order="F")
+    src[0, 5] = np.nan  # lo halo of axis 1
+    dev.source_term = src
+    with pytest.raises(S.InvalidScalarError, match="Invalid source_term in the lo halo region for axis: 1"):
+        S.nonlinear_thermal_conduction_step(dev, mesh, T0.copy(order="F"), 1.0, 1.0, law, 1e-4,
+                                            show_convergence=False)
+    dev.source_term = None
+    with pytest.raises(AssertionError, match="dt > 0"):
+        S.nonlinear_thermal_conduction_step(dev, mesh, T0.copy(order="F"), 1.0, 1.0, law, 0.0,
+                                            show_convergence=False)
+
+
+def test_constant_temperature_errors_like_reference():
+    # an already-converged start gives L2 = 0 at iteration 1, rel = 0/0 = NaN:
+    # the reference raises "Non-finite error detected!" (PseudoTransientSolver.jl:289-307)
+    mesh, _ = gaussian_2d(20)
+    T0 = np.full(mesh.dims_full, 3.0, order="F")
+    law = S.PowerLawConductivity(1.0)
+    dev = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4)
+    ora = O.OracleSolver(mesh, [("neumann", 0.0)] * 4)
+    S.update_conductivity(dev, mesh, T0, 1.0, 1.0, law)
+    ora.update_conductivity(T0)
+    with pytest.raises(O.OracleError, match="Non-finite error detected"):
+        ora.step(T0.copy(order="F"), 1e-4)
+    with pytest.raises(S.NonFiniteError, match="Non-finite error detected"):
+        S.nonlinear_thermal_conduction_step(dev, mesh, T0.copy(order="F"), 1.0, 1.0, law, 1e-4,
+                                            show_convergence=False)
+
+
+def test_apply_bcs_and_next_dt_entry_points():
+    mesh, T0 = gaussian_2d(18, nhalo=4, ni=21)
+    rng = np.random.default_rng(0)
+    u = np.asfortranarray(rng.random(mesh.dims_full))
+    for spec in (["neumann"] * 4, [("dirichlet", 1.0), ("dirichlet", 2.0), ("dirichlet", 3.0), ("dirichlet", 4.0)],
+                 ["periodic", "periodic", "neg", "periodic"]):
+        bd, bo = _bcs(spec)
+        ud = u.copy(order="F")
+        uo = u.copy(order="F")
+        S.applybcs(bd, mesh, ud, 1)
+        O.OracleSolver(mesh, bo).applybcs(uo, 1)
+        np.testing.assert_array_equal(ud, uo)
+    dev = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4)
+    ora = O.OracleSolver(mesh, [("neumann", 0.0)] * 4)
+    a = np.asfortranarray(1 + rng.random(mesh.dims_full))
+    b = np.asfortranarray(a * (1 + 0.01 * rng.standard_normal(mesh.dims_full)))
+    for kw in ({}, {"tol": 1e-2, "maximum_u_change": 0.1, "timestep_growth_factor": 1.5}):
+        assert S.next_dt(a, b, 1e-3, solver=dev, **kw) == ora.next_dt(a, b, 1e-3, **kw)
