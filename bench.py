@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the APT time step on one MI355X.
+
+Workload (default): the 1024^2 "drop-in" configuration -- wavy grid
+x = xi + 0.05 sin(2 pi xi) sin(2 pi eta) (same for y), kappa = T^2.5,
+background T = 0.01 with a seeded hot disk T = 1 of radius 0.05, Dirichlet
+T = 1 on the left edge, zero-flux elsewhere, adaptive dt from next_dt
+(dt0 = 1e-5), residual tolerance 1e-8 (atol, rtol = 0).  A "step" is one
+time step: the full pseudo-transient solve to tolerance.  The temperature
+stays in HBM for the whole timed region.
+
+Prints one JSON line: s/time-step (lower is better), with
+  roofline      the fused sweep kernel's algorithmic bytes / mean duration
+                (HIP events on the solver's stream) against 8 TB/s
+  cpu_baseline  the oracle (CPU restatement of the reference APT, OpenMP)
+                on a bounded sample of the same workload on this host
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {("fp64", 2): 96, ("fp32_dc", 2): 48, ("fp64", 3): 144, ("fp32_dc", 3): 72}
+
+
+def make_workload(n, seed):
+    from cases import hot_disk_2d
+
+    return hot_disk_2d(n, seed=seed)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(args, mesh, T0):
+    from curvilineardiffusionmanuscript_amd import solver as S
+
+    law = S.PowerLawConductivity(1.0, 2.5, 0.0)
+    bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
+    sol = S.PseudoTransientSolver(mesh, bcs)
+    kw = dict(rtol=0.0, atol=args.tol, max_iter=10**7, precision=args.method)
+
+    def reset():
+        sol.reset_state()
+        S.update_conductivity(sol, mesh, T0, 1.0, 1.0, law)
+        sol.set_temperature(T0)
+
+    reset()
+    dt = args.dt0
+    for _ in range(args.warmup):
+        st, dt = sol.step_resident(dt, None, 1.0, law, **kw)
+    reset()
+    dt = args.dt0
+    iters, checks, steps = [], 0, []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st, dt_next = sol.step_resident(dt, None, 1.0, law, **kw)
+        iters.append(st.niter)
+        checks += st.nchecks
+        steps.append((dt, st.niter, st.abs_err, st.time))
+        dt = dt_next
+    elapsed = time.perf_counter() - t0
+    for k, (d, it, err, tm) in enumerate(steps[: min(5, len(steps))]):
+        log(f"step {k}: dt={d:.3e} niter={it} abs_err={err:.3e} time={tm * 1e3:.2f} ms")
+    T_end = sol.get_temperature()
+    # dominant kernel: the fused sweep without residual, timed with HIP events
+    mean_ms = float("nan")
+    if args.sweep_samples > 0:
+        mean_ms = sol_bench(sol, steps[-1][0], args.method, args.sweep_samples)
+    sol.close()
+    return dict(elapsed=elapsed, iters=iters, checks=checks, T_end=T_end, sweep_ms=mean_ms)
+
+
+def sol_bench(sol, dt, method, n):
+    import ctypes as C
+
+    from curvilineardiffusionmanuscript_amd import _lib as L
+
+    out = C.c_double()
+    prec = L.PREC_FP64 if method == "fp64" else L.PREC_FP32_DC
+    rc = L.load().cdiff_bench_sweeps(sol._h, float(dt), prec, 0, int(n), C.byref(out))
+    if rc:
+        raise RuntimeError(L.load().cdiff_last_error(sol._h).decode())
+    return out.value
+
+
+def run_cpu(args, mesh, T0, budget_s):
+    """The oracle on the first steps of the same workload, up to ~budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import apt_oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    O.set_num_threads(threads)
+    s = O.OracleSolver(mesh, [("dirichlet", 1.0)] + [("neumann", 0.0)] * 3, kappa=(1.0, 2.5, 0.0))
+    T = T0.copy(order="F")
+    s.update_conductivity(T)
+    dt = args.dt0
+    t0 = time.perf_counter()
+    nsteps, nit = 0, 0
+    while nsteps < args.steps:
+        st, dt = s.step(T, dt, rtol=0.0, atol=args.tol, max_iter=10**7)
+        nsteps += 1
+        nit += st["niter"]
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return dict(s_per_step=el / nsteps, steps=nsteps, iters=nit, threads=O.num_threads(), elapsed=el)
+
+
+def load_traffic(kernel):
+    p = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1024, help="cells per axis")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--dt0", type=float, default=1e-5)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--method", default="fp64", choices=["fp64", "fp32_dc"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-samples", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: this build has no multi-GPU path (the APT would shard by "
+            "domain decomposition with per-iteration halo exchange; see DESIGN.md). Run with --gpus 1.",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    t_setup = time.perf_counter()
+    mesh, T0 = make_workload(args.n, args.seed)
+    log(f"workload {args.n}^2 built in {time.perf_counter() - t_setup:.1f}s")
+    g = run_gpu(args, mesh, T0)
+    n_cells = args.n * args.n
+    nd = 2
+    bpc = BYTES_PER_CELL[(args.method, nd)]
+    s_per_step = g["elapsed"] / args.steps
+    it_mean = float(np.mean(g["iters"]))
+    teff = bpc * n_cells * sum(g["iters"]) / g["elapsed"] / 1e9
+    log(f"GPU: {s_per_step * 1e3:.3f} ms/step, {it_mean:.1f} its/step, T_eff(step) {teff:.1f} GB/s, "
+        f"sweep {g['sweep_ms'] * 1e3:.1f} us")
+
+    kernel = f"sweep2d_{args.method}"
+    achieved = bpc * n_cells / (g["sweep_ms"] / 1e3) / 1e9 if g["sweep_ms"] == g["sweep_ms"] else None
+    traffic = load_traffic(kernel)
+    roofline = {
+        "kernel": kernel,
+        "bound": "hbm",
+        "achieved": round(achieved, 1) if achieved else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+        "traffic": traffic.get("bytes_per_launch") if traffic else None,
+        "algorithmic_bytes_per_launch": bpc * n_cells,
+        "mean_launch_us": round(g["sweep_ms"] * 1e3, 2),
+    }
+    cpu = None
+    if not args.no_cpu:
+        c = run_cpu(args, mesh, T0, args.cpu_seconds)
+        log(f"CPU oracle: {c['s_per_step']:.3f} s/step over {c['steps']} steps, {c['threads']} threads")
+        cpu = {
+            "value": round(c["s_per_step"], 5),
+            "unit": "s/time-step",
+            "cores": c["threads"],
+            "kind": "port",
+            "sample": f"first {c['steps']} of the {args.steps} steps ({c['iters']} pseudo-iterations), "
+            f"same inputs and tolerance, oracle/apt_oracle.c with OpenMP",
+        }
+    out = {
+        "metric": "s/time-step (APT to 1e-8 FP64 residual) + T_eff %HBM peak",
+        "value": round(s_per_step, 6),
+        "unit": "s/time-step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s_per_step * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.method == "fp64" else "f32+f64",
+        "data": "synthetic (seeded hot disk, analytic wavy mesh)",
+        "config": {
+            "workload": f"2D {args.n}x{args.n} wavy grid (A=0.05), kappa=T^2.5, hot disk seed {args.seed}, "
+            f"Dirichlet T=1 left, adaptive dt0={args.dt0:g}, atol={args.tol:g}, method={args.method}",
+            "iters_per_step": round(it_mean, 2),
+            "t_eff_step_GBs": round(teff, 1),
+            "t_eff_step_frac": round(teff / HBM_PEAK_GBS, 4),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -42,6 +42,7 @@ EXPORTS = (
     "cdiff_get_last_iter",
     "cdiff_set_last_iter",
     "cdiff_reset_state",
+    "cdiff_bench_sweeps",
     "cdiff_last_error",
     "cdiff_status_string",
     "cdiff_destroy",
@@ -126,15 +127,12 @@ def load(path: str = LIB_PATH):
     L.cdiff_get_last_iter.argtypes = [_vp, C.POINTER(C.c_int32)]
     L.cdiff_set_last_iter.argtypes = [_vp, C.c_int32]
     L.cdiff_reset_state.argtypes = [_vp]
+    L.cdiff_bench_sweeps.argtypes = [_vp, C.c_double, C.c_int, C.c_int, C.c_int, _dp]
     L.cdiff_last_error.argtypes = [_vp]
     L.cdiff_last_error.restype = C.c_char_p
     L.cdiff_status_string.argtypes = [C.c_int]
     L.cdiff_status_string.restype = C.c_char_p
     L.cdiff_destroy.argtypes = [_vp]
-    for name in EXPORTS:
-        fn = getattr(L, name)
-        if fn.restype is C.c_int or fn.restype is None and name != "cdiff_default_opts":
-            pass
     for name in EXPORTS:
         if name not in ("cdiff_default_opts", "cdiff_last_error", "cdiff_status_string"):
             getattr(L, name).restype = C.c_int

@@ -473,6 +473,11 @@ int cdiff_set_field(cdiff_solver* h, int field, const double* in) {
   return h->fail(CDIFF_BAD_ARG, "field cannot be set");
 }
 
+int cdiff_bench_sweeps(cdiff_solver* h, double dt, int precision, int check, int nsweeps, double* mean_ms) {
+  if (!h || !mean_ms) return CDIFF_BAD_ARG;
+  return h->bench_sweeps(dt, precision, check, nsweeps, mean_ms);
+}
+
 int cdiff_get_field(cdiff_solver* h, int field, double* out) {
   if (!h || !out) return CDIFF_BAD_ARG;
   return h->get_field(field, out);

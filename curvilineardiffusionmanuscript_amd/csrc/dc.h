@@ -1,0 +1,21 @@
+// dc.h -- kernels of the FP32/FP64 defect-correction step.
+#pragma once
+#include "cdiff_dev.h"
+
+namespace cdiff {
+
+struct DcCoefs {
+  const double* c[9];
+};
+
+__global__ void k_dc_update(Geo g, Ghost gh, KappaLaw kap, const double* rho, const double* u_old, const float* e,
+                            double* u_new, double* a64, float* a32);
+__global__ void k_dc_residual(Geo g, int harmonic, double dt, const double* u, const double* a, const double* uprev,
+                              DcCoefs cf, const double* S, const double* tzA, const double* tzB, float* r32,
+                              double* psum, double* pmax);
+__global__ void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* psum, int nparts, double nfull, double eta,
+                                 int iter, int cap);
+__global__ void k_fill32(int64_t n, float v, float* dst);
+__global__ void k_to_f32(int64_t n, const double* src, float* dst);
+
+}  // namespace cdiff

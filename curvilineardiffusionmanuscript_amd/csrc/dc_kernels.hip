@@ -1,0 +1,220 @@
+// dc_kernels.hip -- FP64 kernels of the mixed-precision defect-correction step.
+//
+// The outer loop keeps the iterate u in FP64 and evaluates the reference's
+// residual r(u) = -mask*(u - u_prev)/dt - div(q'(u)) + S (+ f)
+// (src/pseudo_transient/residuals.jl:24-47, flux_divergence.jl) in FP64; the
+// inner loop runs FP32 APT sweeps on the correction equation
+//   e/dt + div(q'_alpha(e)) = r,  alpha frozen at u,
+// and the outer loop applies u <- u + e.  These are the FP64 pieces:
+//   k_dc_update    u_new = u_old + e on the domain, ghosts = BCs of u_new,
+//                  alpha(u_new) in FP64 (scratch) and FP32 (inner sweeps)
+//   k_dc_residual  r(u_new) per domain cell, stored in FP32 for the inner
+//                  solve, with per-block partials of sum(r^2) and max|r|
+#include <limits>
+
+#include "cdiff_dev.h"
+#include "dc.h"
+
+namespace cdiff {
+
+namespace {
+
+__device__ __forceinline__ void unflat3(const Geo& g, int64_t e, int& i, int& j, int& k) {
+  i = (int)(e % g.n[0]);
+  const int64_t r = e / g.n[0];
+  j = (int)(r % g.n[1]);
+  k = (int)(r / g.n[1]);
+}
+
+}  // namespace
+
+// One thread per cell of the full array.  Domain cells: u_new = u_old + e.
+// First-layer ghost cells on a face: the BC rule applied to u_new (reading
+// u_old + e of the source cell).  Everything else is copied.
+__global__ void k_dc_update(Geo g, Ghost gh, KappaLaw kap, const double* __restrict__ rho,
+                            const double* __restrict__ u_old, const float* __restrict__ e, double* __restrict__ u_new,
+                            double* __restrict__ a64, float* __restrict__ a32) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t ntot = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  if (t >= ntot) return;
+  int i, j, k;
+  unflat3(g, t, i, j, k);
+  const int idx[3] = {i, j, k};
+  int nout = 0, ax = 0, layer = 0;
+  for (int d = 0; d < g.ndim; ++d) {
+    if (idx[d] < g.lo[d]) {
+      ++nout;
+      ax = d;
+      layer = g.lo[d] - idx[d];
+    } else if (idx[d] > g.hi[d]) {
+      ++nout;
+      ax = d;
+      layer = idx[d] - g.hi[d];
+    }
+  }
+  const int64_t c = at(g, i, j, k);
+  double v;
+  if (nout == 0) {
+    v = u_old[c] + (e ? (double)e[c] : 0.0);
+  } else if (nout == 1 && layer == 1) {
+    const int f = 2 * ax + (idx[ax] < g.lo[ax] ? 0 : 1);
+    const int m = gh.mode[f];
+    if (m == G_CONST) {
+      v = gh.val[f];
+    } else if (m == G_KEEP) {
+      v = u_old[c];
+    } else {
+      int s[3] = {i, j, k};
+      s[ax] = gh.src[f];
+      const int64_t sc = at(g, s[0], s[1], s[2]);
+      const double us = u_old[sc] + (e ? (double)e[sc] : 0.0);
+      v = m == G_NEG ? -us : us;
+    }
+  } else {
+    v = u_old[c];
+  }
+  u_new[c] = v;
+  const double a = alpha_of<double>(kap, rho ? rho[c] : 1.0, v);
+  a64[c] = a;
+  a32[c] = (float)a;
+}
+
+// Plain flux q' of the face a+1/2 of cell (i, j, k) along `axis`, zero outside
+// expand_lower(domain, axis) (fluxes_cpu.jl:40-41).
+__device__ __forceinline__ double qp_face(const Geo& g, int harmonic, const double* __restrict__ u,
+                                          const double* __restrict__ a, int axis, int i, int j, int k) {
+  const int idx[3] = {i, j, k};
+  for (int d = 0; d < g.ndim; ++d) {
+    const int lo = g.lo[d] - (d == axis ? 1 : 0);
+    if (idx[d] < lo || idx[d] > g.hi[d]) return 0.0;
+  }
+  const int64_t c = at(g, i, j, k);
+  const int64_t st = axis == 0 ? 1 : (axis == 1 ? g.pitch : g.plane);
+  const double abar = face_mean<double>(harmonic, a[c], a[c + st]);
+  return -abar * (u[c + st] - u[c]);
+}
+
+__global__ __launch_bounds__(256) void k_dc_residual(Geo g, int harmonic, double dt, const double* __restrict__ u,
+                                                     const double* __restrict__ a, const double* __restrict__ uprev,
+                                                     DcCoefs cf, const double* __restrict__ S,
+                                                     const double* __restrict__ tzA, const double* __restrict__ tzB,
+                                                     float* __restrict__ r32, double* __restrict__ psum,
+                                                     double* __restrict__ pmax) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t ntot = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  double sq = 0.0, mx = 0.0;
+  if (t < ntot) {
+    int i, j, k;
+    unflat3(g, t, i, j, k);
+    const int64_t c = at(g, i, j, k);
+    bool dom = i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1];
+    if (g.ndim == 3) dom = dom && k >= g.lo[2] && k <= g.hi[2];
+    if (!dom) {
+      r32[c] = 0.0f;
+    } else {
+      double D;
+      if (g.ndim == 2) {
+        const double xc = qp_face(g, harmonic, u, a, 0, i, j, 0), xw = qp_face(g, harmonic, u, a, 0, i - 1, j, 0);
+        const double xn = qp_face(g, harmonic, u, a, 0, i, j + 1, 0);
+        const double xnw = qp_face(g, harmonic, u, a, 0, i - 1, j + 1, 0);
+        const double xs = qp_face(g, harmonic, u, a, 0, i, j - 1, 0);
+        const double xsw = qp_face(g, harmonic, u, a, 0, i - 1, j - 1, 0);
+        const double yc = qp_face(g, harmonic, u, a, 1, i, j, 0), ys = qp_face(g, harmonic, u, a, 1, i, j - 1, 0);
+        const double ye = qp_face(g, harmonic, u, a, 1, i + 1, j, 0);
+        const double yes = qp_face(g, harmonic, u, a, 1, i + 1, j - 1, 0);
+        const double yw = qp_face(g, harmonic, u, a, 1, i - 1, j, 0);
+        const double yws = qp_face(g, harmonic, u, a, 1, i - 1, j - 1, 0);
+        const double t1 = cf.c[0][c] * (xc - xw);
+        const double t2 = cf.c[1][c] * (yc - ys);
+        const double t3 = cf.c[2][c] * ((xn + xnw) - (xs + xsw));
+        const double t4 = cf.c[2][c] * ((ye + yes) - (yw + yws));
+        const double t5 = cf.c[3][c] * (0.5 * (xc + xw));
+        const double t6 = cf.c[4][c] * (0.5 * (yc + ys));
+        D = t1 + t2 + t3 + t4 + t5 + t6;
+      } else {
+        auto X = [&](int di, int dj, int dk) { return qp_face(g, harmonic, u, a, 0, i + di, j + dj, k + dk); };
+        auto Y = [&](int di, int dj, int dk) { return qp_face(g, harmonic, u, a, 1, i + di, j + dj, k + dk); };
+        auto Z = [&](int di, int dj, int dk) { return qp_face(g, harmonic, u, a, 2, i + di, j + dj, k + dk); };
+        const double xc = X(0, 0, 0), xw = X(-1, 0, 0), yc = Y(0, 0, 0), ys = Y(0, -1, 0);
+        const double zc = Z(0, 0, 0), zs = Z(0, 0, -1);
+        const double a1 = cf.c[0][c] * (xc - xw);
+        const double a2 = cf.c[1][c] * (yc - ys);
+        const double a3 = cf.c[2][c] * (zc - zs);
+        const double qi_eta = cf.c[3][c] * ((X(0, 1, 0) + X(-1, 1, 0)) - (X(0, -1, 0) + X(-1, -1, 0)));
+        const double qi_zeta = cf.c[4][c] * ((X(0, 0, 1) + X(-1, 0, 1)) - (X(0, 0, -1) + X(-1, 0, -1)));
+        const double qj_xi = cf.c[3][c] * ((Y(1, 0, 0) + Y(1, -1, 0)) - (Y(-1, 0, 0) + Y(-1, -1, 0)));
+        const double qj_zeta = cf.c[5][c] * ((Y(0, 0, 1) + Y(0, -1, 1)) - (Y(0, 0, -1) + Y(0, -1, -1)));
+        const double qk_eta = cf.c[5][c] * ((Z(0, 1, 0) + Z(0, 1, -1)) - (Z(0, -1, 0) + Z(0, -1, -1)));
+        const double qk_xi = cf.c[4][c] * ((Z(1, 0, 0) + Z(1, 0, -1)) - (Z(-1, 0, 0) + Z(-1, 0, -1)));
+        const double m1 = cf.c[6][c] * (0.5 * (xc + xw));
+        const double m2 = cf.c[7][c] * (0.5 * (yc + ys));
+        const double m3 = cf.c[8][c] * (0.5 * (zc + zs));
+        D = a1 + a2 + a3 + qi_eta + qi_zeta + qj_xi + qj_zeta + qk_eta + qk_xi + m1 + m2 + m3;
+      }
+      const double un = u[c], um = uprev[c];
+      double du = un - um;
+      bool approx = un == um;
+      if (!approx && isfinite(un) && isfinite(um))
+        approx = fabs(un - um) <= std::numeric_limits<double>::epsilon() * fmax(fabs(un), fabs(um));
+      if (approx) du = copysign(0.0, du);
+      double r = -du / dt - D + (S ? S[c] : 0.0);
+      if (tzA) r = r + (tzA[c] + tzB[c] * a[c]);
+      r32[c] = (float)r;
+      sq = r * r;
+      mx = fabs(r);
+      if (r != r) mx = r;
+    }
+  }
+  __shared__ double ss[256], sm[256];
+  ss[threadIdx.x] = sq;
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      ss[threadIdx.x] += ss[threadIdx.x + o];
+      sm[threadIdx.x] = nanmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    psum[blockIdx.x] = ss[0];
+    pmax[blockIdx.x] = sm[0];
+  }
+}
+
+// Inner-loop decision: stop the FP32 sweeps once the correction residual has
+// dropped below eta * (the outer FP64 residual), or after `cap` sweeps.
+__global__ __launch_bounds__(256) void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* __restrict__ psum,
+                                                        int nparts, double nfull, double eta, int iter, int cap) {
+  if (cin->done) return;
+  __shared__ double ss[256];
+  double s = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += 256) s += psum[p];
+  ss[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) ss[threadIdx.x] += ss[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double L2 = sqrt(ss[0] / nfull);
+  cin->last_l2 = L2;
+  cin->nchecks += 1;
+  if (!(L2 <= 1e300) || L2 <= eta * cout->last_l2 || iter >= cap) {
+    cin->done = 1;
+    cin->final_iter = iter;
+    cin->status = (L2 == L2) ? 0 : 1;
+  }
+}
+
+__global__ void k_fill32(int64_t n, float v, float* __restrict__ dst) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = v;
+}
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ src, float* __restrict__ dst) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = (float)src[t];
+}
+
+}  // namespace cdiff

@@ -1,0 +1,292 @@
+// fp32dc.cpp -- the mixed-precision time step: FP32 APT sweeps inside an FP64
+// defect-correction loop.
+//
+//   u <- T;  r <- r64(u)                      (FP64 residual of the reference
+//                                              discrete equations, FP64 norm)
+//   while |r| > atol (and rel > rtol):
+//     solve  e/dt + div(q'_alpha(e)) = r     with FP32 APT sweeps, alpha frozen
+//                                              at u, to |r_inner| <= eta |r|
+//     u <- u + e;  r <- r64(u)
+//
+// The outer residual is exactly the quantity the reference tests against its
+// tolerance (PseudoTransientSolver.jl:280-316, residuals.jl:24-47), evaluated
+// in FP64, so the step ends at the same FP64 residual tolerance as the FP64
+// APT.  The inner equation is the frozen-conductivity (Picard) linearisation
+// of the residual around u: the outer iteration converges for the nonlinear
+// kappa through the FP64 residual.  The correction e is small, so FP32 holds
+// it to full relative precision; u itself never goes through FP32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+
+#include "dc.h"
+#include "kernels.h"
+#include "solver.h"
+#include "sweep_args.h"
+
+using namespace cdiff;
+
+#define HIP_TRY(expr)                                 \
+  do {                                                \
+    hipError_t _e = (expr);                           \
+    if (_e != hipSuccess) return hip_fail(_e, #expr); \
+  } while (0)
+
+#define LAUNCH_CHECK(name)                                        \
+  do {                                                            \
+    hipError_t _e = hipGetLastError();                            \
+    if (_e != hipSuccess) return hip_fail(_e, "launch of " name); \
+  } while (0)
+
+namespace {
+
+int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return (v && *v) ? std::atoi(v) : dflt;
+}
+
+double env_double(const char* name, double dflt) {
+  const char* v = std::getenv(name);
+  return (v && *v) ? std::atof(v) : dflt;
+}
+
+dim3 grid_n(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace
+
+int cdiff_solver::ensure_fp32() {
+  auto alloc32 = [&](float** p) -> int {
+    if (*p) return CDIFF_OK;
+    HIP_TRY(hipMalloc((void**)p, sizeof(float) * g.total));
+    owned32.push_back(*p);
+    hipLaunchKernelGGL(k_fill32, grid_n(g.total), dim3(256), 0, stream, g.total, 0.0f, *p);
+    LAUNCH_CHECK("k_fill32");
+    return CDIFF_OK;
+  };
+  int rc;
+  for (int b = 0; b < 2; ++b)
+    if ((rc = alloc32(&u32[b]))) return rc;
+  for (int a = 0; a < ndim; ++a)
+    for (int b = 0; b < 2; ++b)
+      if ((rc = alloc32(&q32[a][b]))) return rc;
+  if ((rc = alloc32(&base32))) return rc;
+  if ((rc = alloc32(&a32))) return rc;
+  const int ncoef = ndim == 2 ? 5 : 9;
+  for (int c = 0; c < ncoef; ++c)
+    if ((rc = alloc32(&coef32[c]))) return rc;
+  if (!a64) {
+    HIP_TRY(hipMalloc((void**)&a64, sizeof(double) * g.total));
+    owned.push_back(a64);
+  }
+  if (!ctrl_in) {
+    HIP_TRY(hipMalloc((void**)&ctrl_in, sizeof(Ctrl)));
+    owned.push_back(ctrl_in);
+  }
+  const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  const int nres = (int)((nfull + 255) / 256);
+  if (nres > dc_nparts) {
+    HIP_TRY(hipMalloc((void**)&dc_psum, sizeof(double) * nres));
+    owned.push_back(dc_psum);
+    HIP_TRY(hipMalloc((void**)&dc_pmax, sizeof(double) * nres));
+    owned.push_back(dc_pmax);
+    dc_nparts = nres;
+  }
+  if (!coefs32_valid) {
+    for (int c = 0; c < ncoef; ++c) {
+      hipLaunchKernelGGL(k_to_f32, grid_n(g.total), dim3(256), 0, stream, g.total, (const double*)coef[c], coef32[c]);
+      LAUNCH_CHECK("k_to_f32");
+    }
+    coefs32_valid = true;
+  }
+  if (rho && !aux32_valid) {
+    if ((rc = alloc32(&rho32))) return rc;
+    hipLaunchKernelGGL(k_to_f32, grid_n(g.total), dim3(256), 0, stream, g.total, (const double*)rho, rho32);
+    LAUNCH_CHECK("k_to_f32");
+  }
+  aux32_valid = true;
+  // ghost rules of the correction: Dirichlet (and untouched) ghosts hold no
+  // correction; copy / periodic / negated copies carry over unchanged
+  gh32 = gh;
+  for (int f = 0; f < 6; ++f) {
+    if (gh32.mode[f] == G_CONST || gh32.mode[f] == G_KEEP) {
+      gh32.mode[f] = G_CONST;
+      gh32.val[f] = 0.0;
+    }
+  }
+  return CDIFF_OK;
+}
+
+int cdiff_solver::launch_sweep32(double dt, double Vpdtau, int src, bool check) {
+  if (!u32[0]) return fail(CDIFF_BAD_ARG, "FP32 buffers not set up (run an FP32 step first)");
+  const int dst = 1 - src;
+  const float* r32 = rho ? rho32 : nullptr;
+  if (ndim == 2) {
+    Sweep2DArgs<float> A{};
+    A.g = g;
+    A.gh = gh32;
+    A.kap = kap;
+    A.ic.L = L;
+    A.ic.Vpdtau = Vpdtau;
+    A.ic.dt = dt;
+    A.harmonic = mean == CDIFF_MEAN_HARMONIC;
+    A.ntiles = ntiles;
+    A.chunks = chunks;
+    A.ch = ch;
+    A.u_src = u32[src];
+    A.u_dst = u32[dst];
+    A.qx_src = q32[0][src];
+    A.qx_dst = q32[0][dst];
+    A.qy_src = q32[1][src];
+    A.qy_dst = q32[1][dst];
+    A.g11 = coef32[0];
+    A.g22 = coef32[1];
+    A.c12 = coef32[2];
+    A.ac = coef32[3];
+    A.bc = coef32[4];
+    A.rho = r32;
+    A.frozen_alpha = a32;
+    A.alpha_only = 1;
+    A.base = base32;
+    A.part_sum = part_sum;
+    A.part_max = part_max;
+    A.ctrl = ctrl_in;
+    launch_sweep2d<float>(A, check, nblocks, stream);
+    LAUNCH_CHECK("sweep2d<float>");
+    return CDIFF_OK;
+  }
+  Sweep3DArgs<float> A{};
+  A.g = g;
+  A.gh = gh32;
+  A.kap = kap;
+  A.ic.L = L;
+  A.ic.Vpdtau = Vpdtau;
+  A.ic.dt = dt;
+  A.harmonic = mean == CDIFF_MEAN_HARMONIC;
+  A.ntiles = ntiles;
+  A.chunks = chunks;
+  A.ch = ch;
+  A.stripsI = strips3[0];
+  A.u_src = u32[src];
+  A.u_dst = u32[dst];
+  A.qx_src = q32[0][src];
+  A.qx_dst = q32[0][dst];
+  A.qy_src = q32[1][src];
+  A.qy_dst = q32[1][dst];
+  A.qz_src = q32[2][src];
+  A.qz_dst = q32[2][dst];
+  for (int t = 0; t < 9; ++t) A.coef[t] = coef32[t];
+  A.rho = r32;
+  A.frozen_alpha = a32;
+  A.alpha_only = 1;
+  A.base = base32;
+  A.part_sum = part_sum;
+  A.part_max = part_max;
+  A.ctrl = ctrl_in;
+  launch_sweep3d<float>(A, check, nblocks, stream);
+  LAUNCH_CHECK("sweep3d<float>");
+  return CDIFF_OK;
+}
+
+// residual of the iterate in u[dstbuf] after u[dstbuf] = u[srcbuf] + e
+int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o) {
+  const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  hipLaunchKernelGGL(k_dc_update, grid_n(nfull), dim3(256), 0, stream, g, gh, kap, (const double*)rho,
+                     (const double*)u[srcbuf], e, u[dstbuf], a64, a32);
+  LAUNCH_CHECK("k_dc_update");
+  DcCoefs cf{};
+  for (int t = 0; t < 9; ++t) cf.c[t] = coef[t];
+  hipLaunchKernelGGL(k_dc_residual, grid_n(nfull), dim3(256), 0, stream, g, (int)(mean == CDIFF_MEAN_HARMONIC), dt,
+                     (const double*)u[dstbuf], (const double*)a64, (const double*)u_prev, cf, (const double*)S,
+                     (const double*)tzA, (const double*)tzB, base32, dc_psum, dc_pmax);
+  LAUNCH_CHECK("k_dc_residual");
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, ctrl, (const double*)dc_psum, (const double*)dc_pmax,
+                     (int)((nfull + 255) / 256), iter, INT_MAX, o.rtol, o.atol, N_full, o.norm_kind, 1);
+  LAUNCH_CHECK("k_finalize");
+  return CDIFF_OK;
+}
+
+int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cdiff_stats* st) {
+  int rc = ensure_fp32();
+  if (rc) return rc;
+  const double eta = env_double("CDIFF_DC_ETA", 1e-3);
+  const int ichk = std::max(1, env_int("CDIFF_DC_CHECK", 8));
+  const int max_outer = std::max(1, env_int("CDIFF_DC_MAX_OUTER", 100));
+  const int fixed_inner = o.inner_iters;
+  const int64_t total_cap = (int64_t)o.max_iter;
+  // u[0] holds T (prologue); the first FP64 residual, of T with its BCs, into u[1]
+  int a = 1;
+  if ((rc = dc_residual(dt, 0, nullptr, a, 1, o))) return rc;
+  HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  int outer = 0;
+  int64_t inner_total = 0;
+  int inner_checks = 0;
+  const int64_t nq = g.total;
+  while (!ctrl_h[2].done) {
+    if (outer >= max_outer) {
+      ctrl_h[2].status = CDIFF_MAX_ITER;
+      break;
+    }
+    // inner FP32 solve for the correction
+    hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, u32[0]);
+    for (int ax = 0; ax < ndim; ++ax)
+      for (int b = 0; b < 2; ++b) hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, q32[ax][b]);
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl_in, 0.0, 0.0);
+    LAUNCH_CHECK("inner reset");
+    const int cap = fixed_inner > 0 ? fixed_inner : (int)std::min<int64_t>(INT_MAX - 1, std::max<int64_t>(1, total_cap - inner_total));
+    int it = 0, seg = 0;
+    bool pending = false;
+    const int seg_len = std::max(ichk, env_int("CDIFF_DC_SEG", 4 * ichk));
+    while (true) {
+      const int s = it + 1;
+      const int e = std::min(cap, s + seg_len - 1);
+      for (int t = s; t <= e; ++t) {
+        const bool chk = fixed_inner > 0 ? (t == cap) : (t % ichk == 0 || t == cap);
+        if ((rc = launch_sweep32(dt, Vpdtau, (t - 1) & 1, chk))) return rc;
+        if (chk) {
+          hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(256), 0, stream, ctrl_in, (const Ctrl*)ctrl,
+                             (const double*)part_sum, nparts, N_full, fixed_inner > 0 ? 0.0 : eta, t, cap);
+          LAUNCH_CHECK("k_inner_finalize");
+        }
+      }
+      it = e;
+      HIP_TRY(hipMemcpyAsync(&ctrl_h[seg & 1], ctrl_in, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipEventRecord(ev[seg & 1], stream));
+      if (pending) {
+        HIP_TRY(hipEventSynchronize(ev[(seg - 1) & 1]));
+        if (ctrl_h[(seg - 1) & 1].done) break;
+      }
+      if (it >= cap) break;
+      pending = true;
+      ++seg;
+    }
+    HIP_TRY(hipMemcpyAsync(&ctrl_h[3], ctrl_in, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    const int ni = ctrl_h[3].final_iter > 0 ? ctrl_h[3].final_iter : it;
+    inner_total += ni;
+    inner_checks += ctrl_h[3].nchecks;
+    const float* e = u32[ni & 1];
+    ++outer;
+    if ((rc = dc_residual(dt, a, e, 1 - a, outer + 1, o))) return rc;
+    a = 1 - a;
+    HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (inner_total > total_cap && !ctrl_h[2].done) {
+      ctrl_h[2].status = CDIFF_MAX_ITER;
+      break;
+    }
+  }
+  Ctrl& c = ctrl_h[2];
+  if (!c.done) c.done = 1;
+  st->rel_err = c.rel;
+  st->abs_err = c.abs_;
+  st->niter = (int)std::min<int64_t>(inner_total, INT_MAX);
+  st->nchecks = c.nchecks + inner_checks;
+  st->outer = outer;
+  c.final_iter = st->niter;
+  dc_final_buf = a;
+  return CDIFF_OK;
+}

@@ -70,16 +70,102 @@ void cdiff_solver::configure_sweep() {
     nblocks = (ntiles + 7) / 8 * 8;
   } else {
     const int nk = g.hi[2] - g.lo[2] + 1;
-    strips3[0] = (ni + 15 - 2) / (16 - 2);
-    strips3[1] = (nj + 16 - 2 - 1) / (16 - 2);
+    strips3[0] = (ni + OWN - 1) / OWN;
+    strips3[1] = (nj + (SWEEP3D_BJ - 2) - 1) / (SWEEP3D_BJ - 2);
     int c = env_int("CDIFF_CH3", 0);
-    if (c <= 0) c = std::min(nk, 32);
+    if (c <= 0) {
+      // aim for ~2048 blocks of SWEEP3D_BJ waves, 4..64 planes per chunk
+      const int64_t cols = (int64_t)strips3[0] * strips3[1];
+      c = (int)std::min<int64_t>(64, std::max<int64_t>(4, (cols * nk + 2047) / 2048));
+    }
+    c = std::min(c, nk);
     ch = c;
     chunks = (nk + ch - 1) / ch;
     ntiles = strips3[0] * strips3[1] * chunks;
     nblocks = (ntiles + 7) / 8 * 8;
   }
   nparts = nblocks;
+}
+
+int cdiff_solver::launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check) {
+  const int dst = 1 - src;
+  if (ndim == 2) {
+    Sweep2DArgs<double> A{};
+    A.g = g;
+    A.gh = gh;
+    A.kap = kap;
+    A.ic.L = L;
+    A.ic.Vpdtau = Vpdtau;
+    A.ic.dt = dt;
+    A.harmonic = mean == CDIFF_MEAN_HARMONIC;
+    A.ntiles = ntiles;
+    A.chunks = chunks;
+    A.ch = ch;
+    A.u_src = u[src];
+    A.u_dst = u[dst];
+    A.qx_src = q[0][src];
+    A.qx_dst = q[0][dst];
+    A.qy_src = q[1][src];
+    A.qy_dst = q[1][dst];
+    A.u_prev = u_prev;
+    A.g11 = coef[0];
+    A.g22 = coef[1];
+    A.c12 = coef[2];
+    A.ac = coef[3];
+    A.bc = coef[4];
+    A.S = S;
+    A.rho = rho;
+    A.tzA = tzA;
+    A.tzB = tzB;
+    if (!o.subcycle_conductivity) {
+      A.frozen_alpha = frozen[0];
+      A.frozen_theta = frozen[1];
+      A.frozen_dtau = frozen[2];
+    }
+    A.part_sum = part_sum;
+    A.part_max = part_max;
+    A.ctrl = ctrl;
+    launch_sweep2d<double>(A, check, nblocks, stream);
+    LAUNCH_CHECK("sweep2d");
+    return CDIFF_OK;
+  }
+  Sweep3DArgs<double> A{};
+  A.g = g;
+  A.gh = gh;
+  A.kap = kap;
+  A.ic.L = L;
+  A.ic.Vpdtau = Vpdtau;
+  A.ic.dt = dt;
+  A.harmonic = mean == CDIFF_MEAN_HARMONIC;
+  A.ntiles = ntiles;
+  A.chunks = chunks;
+  A.ch = ch;
+  A.stripsI = strips3[0];
+  A.u_src = u[src];
+  A.u_dst = u[dst];
+  A.qx_src = q[0][src];
+  A.qx_dst = q[0][dst];
+  A.qy_src = q[1][src];
+  A.qy_dst = q[1][dst];
+  A.qz_src = q[2][src];
+  A.qz_dst = q[2][dst];
+  A.u_prev = u_prev;
+  for (int t = 0; t < 9; ++t) A.coef[t] = coef[t];
+  A.S = S;
+  A.rho = rho;
+  A.tzA = tzA;
+  A.tzB = tzB;
+  if (!o.subcycle_conductivity) {
+    A.frozen_alpha = frozen[0];
+    A.frozen_theta = frozen[1];
+    A.frozen_dtau = frozen[2];
+  }
+  A.part_sum = part_sum;
+  A.part_max = part_max;
+  A.ctrl = ctrl;
+  launch_sweep3d<double>(A, check, nblocks, stream);
+  LAUNCH_CHECK("sweep3d");
+  return CDIFF_OK;
 }
 
 int cdiff_solver::enqueue_next_dt(const double* unew, const double* uold, double dt, double tol, double muc,
@@ -105,36 +191,6 @@ int cdiff_solver::run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int 
   const int lmax = std::max(lmin, env_int("CDIFF_SEG_MAX", 256));
   const int64_t limit = (int64_t)o.max_iter + 1;
 
-  Sweep2DArgs<double> A{};
-  A.g = g;
-  A.gh = gh;
-  A.kap = kap;
-  A.ic.L = L;
-  A.ic.Vpdtau = Vpdtau;
-  A.ic.dt = dt;
-  A.harmonic = mean == CDIFF_MEAN_HARMONIC;
-  A.ntiles = ntiles;
-  A.chunks = chunks;
-  A.ch = ch;
-  A.u_prev = u_prev;
-  A.g11 = coef[0];
-  A.g22 = coef[1];
-  A.c12 = coef[2];
-  A.ac = coef[3];
-  A.bc = coef[4];
-  A.S = S;
-  A.rho = rho;
-  A.tzA = tzA;
-  A.tzB = tzB;
-  if (!o.subcycle_conductivity) {
-    A.frozen_alpha = frozen[0];
-    A.frozen_theta = frozen[1];
-    A.frozen_dtau = frozen[2];
-  }
-  A.part_sum = part_sum;
-  A.part_max = part_max;
-  A.ctrl = ctrl;
-
   int64_t it = 0;
   int seg = 0;
   bool pending = false;
@@ -145,20 +201,9 @@ int cdiff_solver::run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int 
     while (e < cap && !is_check((int)e, m, k)) ++e;
     for (int64_t t = s; t <= e; ++t) {
       const bool chk = is_check((int)t, m, k);
-      const int src = (int)((cur + t - 1) & 1), dst = 1 - src;
-      if (ndim == 2) {
-        A.u_src = u[src];
-        A.u_dst = u[dst];
-        A.qx_src = q[0][src];
-        A.qx_dst = q[0][dst];
-        A.qy_src = q[1][src];
-        A.qy_dst = q[1][dst];
-        launch_sweep2d<double>(A, chk, nblocks, stream);
-        LAUNCH_CHECK("sweep2d");
-      } else {
-        int rc = launch3d_fp64(o, dt, Vpdtau, src, chk);
-        if (rc) return rc;
-      }
+      const int src = (int)((cur + t - 1) & 1);
+      int rc = launch_sweep64(o, dt, Vpdtau, src, chk);
+      if (rc) return rc;
       if (chk || t == limit) {
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, ctrl, (const double*)part_sum,
                            (const double*)part_max, chk ? nparts : 0, (int)t, o.max_iter, o.rtol, o.atol, N_full,
@@ -304,7 +349,8 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   }
 
   const Ctrl& c = ctrl_h[2];
-  if (o.precision == CDIFF_PREC_FP64) {
+  const bool dc = o.precision == CDIFF_PREC_FP32_DC;
+  if (!dc) {
     st.rel_err = c.rel;
     st.abs_err = c.abs_;
     st.niter = c.final_iter;
@@ -329,10 +375,23 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
     return validation_message(status);
   }
   const int fi = c.final_iter;
-  const int src = (int)((cur + fi - 1) & 1), dst = 1 - src;
+  int src = (int)((cur + fi - 1) & 1), dst = 1 - src;
   last_iter = fi;
-  // alpha of the last iteration (for the next step's validate_scalar, :238)
-  if (o.subcycle_conductivity || o.precision != CDIFF_PREC_FP64) {
+  if (dc) {
+    // final iterate (with the BC ghosts of its own domain values) in u[dst];
+    // alpha of that state; the FP64 damped flux restarts from the plain flux
+    // of that state (the APT fixed point of q), so a following FP64 step
+    // warm-starts consistently
+    dst = dc_final_buf;
+    src = 1 - dst;
+    hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)u[dst],
+                       u[src]);
+    hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)a64, alpha);
+    for (int ax = 0; ax < ndim; ++ax)
+      hipLaunchKernelGGL(k_plain_flux, grid_full(g), dim3(256), 0, stream, g, ax, (int)(mean == CDIFF_MEAN_HARMONIC),
+                         (const double*)u[dst], (const double*)a64, q[ax][dst]);
+    LAUNCH_CHECK("dc epilogue");
+  } else if (o.subcycle_conductivity) {
     hipLaunchKernelGGL(k_alpha_full, grid_full(g), dim3(256), 0, stream, g, kap, (const double*)rho,
                        (const double*)u[src], (const double*)u[dst], alpha);
   } else {
@@ -462,4 +521,40 @@ int cdiff_solver::get_field(int field, double* out) {
   rc = dl(res);
   release();
   return rc;
+}
+
+int cdiff_solver::bench_sweeps(double dt, int precision, int check, int n, double* mean_ms) {
+  if (!metrics_set) return fail(CDIFF_NOT_READY, "cdiff_set_metrics has not been called");
+  if (n < 1 || !(dt > 0)) return fail(CDIFF_BAD_ARG, "nsweeps >= 1 and dt > 0 required");
+  cdiff_opts o;
+  cdiff_default_opts(&o);
+  double smin = spacing[0];
+  for (int a = 1; a < ndim; ++a) smin = std::min(smin, spacing[a]);
+  const double Vpdtau = o.cfl * smin;
+  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, 0.0, 0.0);
+  LAUNCH_CHECK("k_ctrl_reset");
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int rc = CDIFF_OK;
+  (void)hipEventRecord(e0, stream);
+  for (int t = 0; t < n && rc == CDIFF_OK; ++t) {
+    const int src = (cur + t) & 1;
+    if (precision == CDIFF_PREC_FP64)
+      rc = launch_sweep64(o, dt, Vpdtau, src, check != 0);
+    else
+      rc = launch_sweep32(dt, Vpdtau, src, check != 0);
+  }
+  (void)hipEventRecord(e1, stream);
+  hipError_t se = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (se != hipSuccess) return hip_fail(se, "hipEventSynchronize");
+  cur = (cur + n) & 1;
+  have_last = false;
+  *mean_ms = ms / n;
+  return CDIFF_OK;
 }

@@ -48,7 +48,14 @@ struct cdiff_solver {
   float* base32 = nullptr;  // u_prev / Delta-t + S + correction, per cell
   float* coef32[9] = {nullptr};
   float *S32 = nullptr, *rho32 = nullptr, *tzA32 = nullptr, *tzB32 = nullptr;
-  float* frozen32[3] = {nullptr, nullptr, nullptr};
+  float* a32 = nullptr;     // alpha frozen at the FP64 iterate
+  double* a64 = nullptr;    // alpha of the FP64 iterate
+  cdiff::Ghost gh32{};      // ghost rules of the correction
+  cdiff::Ctrl* ctrl_in = nullptr;
+  double* dc_psum = nullptr;
+  double* dc_pmax = nullptr;
+  int dc_nparts = 0;
+  int dc_final_buf = 0;
   std::vector<void*> owned32;
   bool coefs32_valid = false, aux32_valid = false, q32_valid = false;
 
@@ -78,5 +85,9 @@ struct cdiff_solver {
   int get_field(int field, double* out);
   int validation_message(int status);
   int step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cdiff_stats* stats);
-  int launch3d_fp64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check);
+  int launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check);
+  int launch_sweep32(double dt, double Vpdtau, int src, bool check);
+  int bench_sweeps(double dt, int precision, int check, int n, double* mean_ms);
+  int ensure_fp32();
+  int dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o);
 };

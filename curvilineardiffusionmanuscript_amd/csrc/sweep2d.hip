@@ -74,8 +74,13 @@ __device__ __forceinline__ RowVals<T> load_row(const Sweep2DArgs<T>& A, int i, i
   v.u = ghost_u(A, i, r);
   if (A.frozen_alpha) {
     v.a = A.frozen_alpha[c];
-    v.th = A.frozen_theta[c];
-    v.dt = A.frozen_dtau[c];
+    if (A.alpha_only) {
+      const T rho = A.rho ? A.rho[c] : T(1);
+      iter_params<T>(A.ic, rho, v.a, v.dt, v.th);
+    } else {
+      v.th = A.frozen_theta[c];
+      v.dt = A.frozen_dtau[c];
+    }
   } else {
     const T rho = A.rho ? A.rho[c] : T(1);
     v.a = alpha_of<T>(A.kap, rho, v.u);
@@ -241,23 +246,33 @@ __global__ __launch_bounds__(64) void sweep2d_kernel(Sweep2DArgs<T> A) {
     if (dom_col) {
       const int64_t c = at(g, i, j, 0);
       const T g11 = A.g11[c], g22 = A.g22[c], c12 = A.c12[c], ac = A.ac[c], bc = A.bc[c];
-      const T up = A.u_prev[c];
-      const T S = A.S ? A.S[c] : T(0);
-      T f = T(0);
-      if (A.tzA) f = A.tzA[c] + A.tzB[c] * rc.a;
       const T D = divergence<T>(g11, g22, c12, ac, bc, x_c, x_w, x_n, x_nw, x_s, x_sw, y_c, y_s, y_e, y_es, y_w, y_ws);
-      T rhs = up / dt - D + S;
-      if (A.tzA) rhs = rhs + f;
-      const T un = (rc.u + rc.dt * rhs) / (T(1) + rc.dt / dt);
+      T un, base = T(0), up = T(0), S = T(0), f = T(0);
+      if (A.base) {
+        base = A.base[c];
+        un = (rc.u + rc.dt * (base - D)) / (T(1) + rc.dt / dt);
+      } else {
+        up = A.u_prev[c];
+        S = A.S ? A.S[c] : T(0);
+        if (A.tzA) f = A.tzA[c] + A.tzB[c] * rc.a;
+        T rhs = up / dt - D + S;
+        if (A.tzA) rhs = rhs + f;
+        un = (rc.u + rc.dt * rhs) / (T(1) + rc.dt / dt);
+      }
       udst[c] = un;
       qyd[c] = y_c;
       if (CHECK) {
         const T Dp =
             divergence<T>(g11, g22, c12, ac, bc, px_c, px_w, px_n, px_nw, px_s, px_sw, py_c, py_s, py_e, py_es, py_w, py_ws);
-        T du = un - up;
-        if (jl_isapprox<T>(un, up, eps)) du = copysign(T(0), du);
-        T res = -du / dt - Dp + S;
-        if (A.tzA) res = res + f;
+        T res;
+        if (A.base) {
+          res = base - un / dt - Dp;
+        } else {
+          T du = un - up;
+          if (jl_isapprox<T>(un, up, eps)) du = copysign(T(0), du);
+          res = -du / dt - Dp + S;
+          if (A.tzA) res = res + f;
+        }
         const double rd = (double)res;
         acc += rd * rd;
         amax = nanmax(amax, fabs(rd));

@@ -31,10 +31,57 @@ struct Sweep2DArgs {
   const T* __restrict__ frozen_alpha;  // nullable: subcycle_conductivity = false
   const T* __restrict__ frozen_theta;
   const T* __restrict__ frozen_dtau;
+  // correction mode (defect correction): the state is the correction e, the
+  // right-hand side `base` replaces u_prev/dt + S, and theta/dtau come from
+  // the frozen alpha (alpha_only) instead of stored arrays
+  const T* __restrict__ base;
+  int alpha_only;
   double* part_sum;
   double* part_max;
   const Ctrl* ctrl;
 };
+
+#ifndef SWEEP3D_BJ
+#define SWEEP3D_BJ 8  // waves (rows) per 3D block; rows 1..BJ-2 are owned
+#endif
+
+template <typename T>
+struct Sweep3DArgs {
+  Geo g;
+  Ghost gh;
+  KappaLaw kap;
+  IterConst ic;
+  int harmonic;
+  int ntiles, chunks, ch, stripsI;
+  const T* __restrict__ u_src;
+  T* __restrict__ u_dst;
+  const T* __restrict__ qx_src;
+  T* __restrict__ qx_dst;
+  const T* __restrict__ qy_src;
+  T* __restrict__ qy_dst;
+  const T* __restrict__ qz_src;
+  T* __restrict__ qz_dst;
+  const T* __restrict__ u_prev;
+  const T* coef[9];  // g11 g22 g33, 0.25*g_xi.eta, 0.25*g_xi.zeta, 0.25*g_eta.zeta, alpha_c beta_c gamma_c
+  const T* __restrict__ S;
+  const T* __restrict__ rho;
+  const T* __restrict__ tzA;
+  const T* __restrict__ tzB;
+  const T* __restrict__ frozen_alpha;
+  const T* __restrict__ frozen_theta;
+  const T* __restrict__ frozen_dtau;
+  // correction mode (defect correction): the state is the correction e, the
+  // right-hand side `base` replaces u_prev/dt + S, and theta/dtau come from
+  // the frozen alpha (alpha_only) instead of stored arrays
+  const T* __restrict__ base;
+  int alpha_only;
+  double* part_sum;
+  double* part_max;
+  const Ctrl* ctrl;
+};
+
+template <typename T>
+void launch_sweep3d(const Sweep3DArgs<T>& A, bool check, int nblocks, hipStream_t s);
 
 template <typename T>
 void launch_sweep2d(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s);

@@ -211,6 +211,14 @@ int cdiff_set_last_iter(cdiff_solver *h, int32_t v);
 /* Zero q and q', set last_err_check = 0 (a freshly constructed solver). */
 int cdiff_reset_state(cdiff_solver *h);
 
+/* Measurement helper (no counterpart in the reference): run `nsweeps`
+ * back-to-back fused sweeps of the current state on the handle's stream
+ * (FP64, or the FP32 correction sweep with precision = CDIFF_PREC_FP32_DC),
+ * with (check = 1) or without the residual, bracketed by HIP events; returns
+ * the mean kernel time in milliseconds.  Clobbers the iterate: call it after
+ * the steps being measured. */
+int cdiff_bench_sweeps(cdiff_solver *h, double dt, int precision, int check, int nsweeps, double *mean_ms);
+
 const char *cdiff_last_error(const cdiff_solver *h);
 const char *cdiff_status_string(int status);
 int cdiff_destroy(cdiff_solver *h);
