@@ -153,7 +153,10 @@ int cdiff_solver::launch_sweep32(double dt, double Vpdtau, int src, bool check) 
     A.part_sum = part_sum;
     A.part_max = part_max;
     A.ctrl = ctrl_in;
-    launch_sweep2d<float>(A, check, nblocks, stream);
+    if (tile2d)
+      launch_sweep2d_tile<float>(A, check, nblocks, stream);
+    else
+      launch_sweep2d<float>(A, check, nblocks, stream);
     LAUNCH_CHECK("sweep2d<float>");
     return CDIFF_OK;
   }

@@ -54,9 +54,20 @@ int env_int(const char* name, int dflt) {
 }  // namespace
 
 void cdiff_solver::configure_sweep() {
+  {
+    const char* v = std::getenv("CDIFF_SWEEP2D");
+    tile2d = !(v && std::string(v) == "march");
+  }
   const int ni = g.hi[0] - g.lo[0] + 1;
   const int nj = g.hi[1] - g.lo[1] + 1;
-  if (ndim == 2) {
+  if (ndim == 2 && tile2d) {
+    // LDS-tiled kernel: 64 x 4 cells per block
+    strips = (ni + 63) / 64;
+    ch = 4;
+    chunks = (nj + 3) / 4;
+    ntiles = strips * chunks;
+    nblocks = (ntiles + 7) / 8 * 8;
+  } else if (ndim == 2) {
     strips = (ni + OWN - 1) / OWN;
     int c = env_int("CDIFF_CH", 0);
     if (c <= 0) {
@@ -125,7 +136,10 @@ int cdiff_solver::launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, 
     A.part_sum = part_sum;
     A.part_max = part_max;
     A.ctrl = ctrl;
-    launch_sweep2d<double>(A, check, nblocks, stream);
+    if (tile2d)
+      launch_sweep2d_tile<double>(A, check, nblocks, stream);
+    else
+      launch_sweep2d<double>(A, check, nblocks, stream);
     LAUNCH_CHECK("sweep2d");
     return CDIFF_OK;
   }
@@ -532,6 +546,7 @@ int cdiff_solver::bench_sweeps(double dt, int precision, int check, int n, doubl
   for (int a = 1; a < ndim; ++a) smin = std::min(smin, spacing[a]);
   const double Vpdtau = o.cfl * smin;
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, 0.0, 0.0);
+  if (ctrl_in) hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl_in, 0.0, 0.0);
   LAUNCH_CHECK("k_ctrl_reset");
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));

@@ -60,6 +60,7 @@ struct cdiff_solver {
   bool coefs32_valid = false, aux32_valid = false, q32_valid = false;
 
   // sweep decomposition
+  bool tile2d = true;  // 2D: LDS-tiled kernel (else the row-marching kernel)
   int strips = 0, chunks = 0, ch = 0, ntiles = 0, nblocks = 0;
   int strips3[2] = {0, 0};
 

@@ -124,6 +124,80 @@ __device__ __forceinline__ bool jl_isapprox(T x, T y, T rtol) {
   return fabs(x - y) <= rtol * m;
 }
 
+// Raw loads of one pipeline stage: row r of the iterate (u with its ghost
+// rule, rho), the old fluxes of the faces built at this stage (xi faces of
+// row r, eta faces between rows r-1 and r), and the cell data of row r-1.
+template <typename T>
+struct Pre {
+  T u, rho, qx, qy;
+  T up, g11, g22, c12, ac, bc, S, tA, tB;
+  T fa, fth, fdt;
+};
+
+template <typename T>
+__device__ __forceinline__ Pre<T> prefetch(const Sweep2DArgs<T>& A, int i, int r, bool active, bool xi_face,
+                                           bool eta_face, bool dom_col) {
+  const Geo& g = A.g;
+  Pre<T> p;
+  p.u = p.rho = p.qx = p.qy = T(0);
+  p.up = p.g11 = p.g22 = p.c12 = p.ac = p.bc = p.S = p.tA = p.tB = T(0);
+  p.fa = p.fth = p.fdt = T(0);
+  if (!active) return p;
+  const int64_t c = at(g, i, r, 0);
+  p.u = ghost_u(A, i, r);
+  if (A.frozen_alpha) {
+    p.fa = A.frozen_alpha[c];
+    if (!A.alpha_only) {
+      p.fth = A.frozen_theta[c];
+      p.fdt = A.frozen_dtau[c];
+    }
+  }
+  if (A.rho) p.rho = A.rho[c];
+  if (xi_face && r <= g.hi[1]) p.qx = A.qx_src[c];
+  if (eta_face) p.qy = A.qy_src[at(g, i, r - 1, 0)];
+  if (dom_col && r - 1 <= g.hi[1]) {
+    const int64_t cm = at(g, i, r - 1, 0);
+    p.g11 = A.g11[cm];
+    p.g22 = A.g22[cm];
+    p.c12 = A.c12[cm];
+    p.ac = A.ac[cm];
+    p.bc = A.bc[cm];
+    if (A.base) {
+      p.up = A.base[cm];
+    } else {
+      p.up = A.u_prev[cm];
+      if (A.S) p.S = A.S[cm];
+      if (A.tzA) {
+        p.tA = A.tzA[cm];
+        p.tB = A.tzB[cm];
+      }
+    }
+  }
+  return p;
+}
+
+template <typename T>
+__device__ __forceinline__ RowVals<T> make_row(const Sweep2DArgs<T>& A, const Pre<T>& p, int i, int r, bool active) {
+  RowVals<T> v;
+  v.u = p.u;
+  v.a = v.th = v.dt = T(0);
+  if (!active) return v;
+  const T rho = A.rho ? p.rho : T(1);
+  if (A.frozen_alpha) {
+    v.a = p.fa;
+    if (A.alpha_only) {
+      iter_params<T>(A.ic, rho, v.a, v.dt, v.th);
+    } else {
+      v.th = p.fth;
+      v.dt = p.fdt;
+    }
+  } else {
+    v.a = alpha_of<T>(A.kap, rho, v.u);
+    iter_params<T>(A.ic, rho, v.a, v.dt, v.th);
+  }
+  return v;
+}
+
 template <typename T, bool CHECK>
 __global__ __launch_bounds__(64) void sweep2d_kernel(Sweep2DArgs<T> A) {
   if (A.ctrl->done) return;
@@ -212,9 +286,15 @@ __global__ __launch_bounds__(64) void sweep2d_kernel(Sweep2DArgs<T> A) {
 
   double acc = 0.0, amax = 0.0;
 
+  // Software pipeline: the loads of row j+2 (and of the cell data of row j+1)
+  // are issued before the arithmetic of row j+1 / cell row j, so memory
+  // latency overlaps the FP64 work.
+  Pre<T> pn = prefetch(A, i, y0 + 1, active, xi_face, eta_face, dom_col);
   for (int j = y0; j <= y1; ++j) {
     const int r = j + 1;
-    RowVals<T> rn = load_row(A, i, r, active);
+    const Pre<T> pc = pn;
+    if (j + 1 <= y1) pn = prefetch(A, i, j + 2, active, xi_face, eta_face, dom_col);
+    const RowVals<T> rn = make_row(A, pc, i, r, active);
     RowVals<T> rnR;
     rnR.u = from_right(rn.u);
     rnR.a = from_right(rn.a);
@@ -223,12 +303,12 @@ __global__ __launch_bounds__(64) void sweep2d_kernel(Sweep2DArgs<T> A) {
     {
       T qn, qp;
       if (xi_face && r <= g.hi[1]) {
-        face_flux<T>(harm, A.qx_src[at(g, i, r, 0)], rnR, rn, qn, qp);
+        face_flux<T>(harm, pc.qx, rnR, rn, qn, qp);
         x_n = qn;
         px_n = qp;
       }
       if (eta_face) {
-        face_flux<T>(harm, A.qy_src[at(g, i, j, 0)], rn, rc, qn, qp);
+        face_flux<T>(harm, pc.qy, rn, rc, qn, qp);
         y_c = qn;
         py_c = qp;
       }
@@ -245,16 +325,16 @@ __global__ __launch_bounds__(64) void sweep2d_kernel(Sweep2DArgs<T> A) {
 
     if (dom_col) {
       const int64_t c = at(g, i, j, 0);
-      const T g11 = A.g11[c], g22 = A.g22[c], c12 = A.c12[c], ac = A.ac[c], bc = A.bc[c];
-      const T D = divergence<T>(g11, g22, c12, ac, bc, x_c, x_w, x_n, x_nw, x_s, x_sw, y_c, y_s, y_e, y_es, y_w, y_ws);
+      const T D = divergence<T>(pc.g11, pc.g22, pc.c12, pc.ac, pc.bc, x_c, x_w, x_n, x_nw, x_s, x_sw, y_c, y_s, y_e,
+                                y_es, y_w, y_ws);
       T un, base = T(0), up = T(0), S = T(0), f = T(0);
       if (A.base) {
-        base = A.base[c];
+        base = pc.up;
         un = (rc.u + rc.dt * (base - D)) / (T(1) + rc.dt / dt);
       } else {
-        up = A.u_prev[c];
-        S = A.S ? A.S[c] : T(0);
-        if (A.tzA) f = A.tzA[c] + A.tzB[c] * rc.a;
+        up = pc.up;
+        S = pc.S;
+        if (A.tzA) f = pc.tA + pc.tB * rc.a;
         T rhs = up / dt - D + S;
         if (A.tzA) rhs = rhs + f;
         un = (rc.u + rc.dt * rhs) / (T(1) + rc.dt / dt);
@@ -262,8 +342,8 @@ __global__ __launch_bounds__(64) void sweep2d_kernel(Sweep2DArgs<T> A) {
       udst[c] = un;
       qyd[c] = y_c;
       if (CHECK) {
-        const T Dp =
-            divergence<T>(g11, g22, c12, ac, bc, px_c, px_w, px_n, px_nw, px_s, px_sw, py_c, py_s, py_e, py_es, py_w, py_ws);
+        const T Dp = divergence<T>(pc.g11, pc.g22, pc.c12, pc.ac, pc.bc, px_c, px_w, px_n, px_nw, px_s, px_sw, py_c,
+                                   py_s, py_e, py_es, py_w, py_ws);
         T res;
         if (A.base) {
           res = base - un / dt - Dp;

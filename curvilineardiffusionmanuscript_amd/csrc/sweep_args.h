@@ -84,6 +84,9 @@ template <typename T>
 void launch_sweep3d(const Sweep3DArgs<T>& A, bool check, int nblocks, hipStream_t s);
 
 template <typename T>
+void launch_sweep2d_tile(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s);
+
+template <typename T>
 void launch_sweep2d(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s);
 
 }  // namespace cdiff
