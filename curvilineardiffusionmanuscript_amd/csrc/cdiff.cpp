@@ -141,6 +141,12 @@ static void set_law(cdiff_solver* h, const cdiff_kappa* kappa, double cp) {
   auto int_exp = [](double p) { return (p == std::floor(p) && p >= 0 && p <= 3) ? (int)p : -1; };
   k.t_int = int_exp(kappa->t_exp);
   k.rho_int = int_exp(kappa->rho_exp);
+  // T^(n+1/2) by x^n*sqrt(x) unless the exact pow() is requested
+  const char* ex = std::getenv("CDIFF_EXACT_POW");
+  const double f2 = 2.0 * kappa->t_exp;
+  const bool half = f2 == std::floor(f2) && kappa->t_exp != std::floor(kappa->t_exp) && kappa->t_exp > 0 &&
+                    kappa->t_exp < 4;
+  k.t_half = (half && !(ex && *ex == '1')) ? (int)std::floor(kappa->t_exp) : -1;
   h->cp = cp;
 }
 

@@ -78,7 +78,19 @@ __device__ __forceinline__ T lit_pow(T x, T p, int p_int) {
 struct KappaLaw {
   double kappa0, t_exp, rho_exp, cp;
   int t_int, rho_int;  // integer exponent 0..3, or -1 for pow()
+  int t_half;          // t_exp = t_half + 1/2 (0..3): x^t_half * sqrt(x), or -1
 };
+
+// |x|^(n + 1/2) as a literal power times a square root: within 2 ulp of
+// pow(x, n + 0.5) at a fraction of its cost.
+template <typename T>
+__device__ __forceinline__ T half_pow(T x, int n) {
+  const T r = sqrt(x);
+  if (n == 0) return r;
+  if (n == 1) return x * r;
+  if (n == 2) return (x * x) * r;
+  return (x * x * x) * r;
+}
 
 // alpha = |kappa(rho, T) / (rho cp)| (src/pseudo_transient/conductivity.jl:7-10).
 template <typename T>
@@ -87,7 +99,7 @@ __device__ __forceinline__ T alpha_of(const KappaLaw& k, T rho, T u) {
   if (!isfinite(u)) {
     kap = T(0);
   } else {
-    T tt = lit_pow<T>(fabs(u), (T)k.t_exp, k.t_int);
+    T tt = k.t_half >= 0 ? half_pow<T>(fabs(u), k.t_half) : lit_pow<T>(fabs(u), (T)k.t_exp, k.t_int);
     if (k.rho_exp == 0.0) {
       kap = (T)k.kappa0 * tt;
     } else {

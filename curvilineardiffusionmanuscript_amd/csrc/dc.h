@@ -9,7 +9,7 @@ struct DcCoefs {
 };
 
 __global__ void k_dc_update(Geo g, Ghost gh, KappaLaw kap, const double* rho, const double* u_old, const float* e,
-                            double* u_new, double* a64, float* a32);
+                            double* u_new, double* a64, float* a32, float* ub32, float* da32);
 __global__ void k_dc_residual(Geo g, int harmonic, double dt, const double* u, const double* a, const double* uprev,
                               DcCoefs cf, const double* S, const double* tzA, const double* tzB, float* r32,
                               double* psum, double* pmax);

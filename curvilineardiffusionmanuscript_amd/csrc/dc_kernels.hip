@@ -33,7 +33,8 @@ __device__ __forceinline__ void unflat3(const Geo& g, int64_t e, int& i, int& j,
 // u_old + e of the source cell).  Everything else is copied.
 __global__ void k_dc_update(Geo g, Ghost gh, KappaLaw kap, const double* __restrict__ rho,
                             const double* __restrict__ u_old, const float* __restrict__ e, double* __restrict__ u_new,
-                            double* __restrict__ a64, float* __restrict__ a32) {
+                            double* __restrict__ a64, float* __restrict__ a32, float* __restrict__ ub32,
+                            float* __restrict__ da32) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t ntot = (int64_t)g.n[0] * g.n[1] * g.n[2];
   if (t >= ntot) return;
@@ -77,6 +78,12 @@ __global__ void k_dc_update(Geo g, Ghost gh, KappaLaw kap, const double* __restr
   const double a = alpha_of<double>(kap, rho ? rho[c] : 1.0, v);
   a64[c] = a;
   a32[c] = (float)a;
+  if (ub32) {
+    // d(alpha)/du of kappa0 |rho|^a |u|^b / (rho cp) is b * alpha / u
+    ub32[c] = (float)v;
+    const double d = (v != 0.0 && isfinite(v)) ? kap.t_exp * a / v : 0.0;
+    da32[c] = (float)(isfinite(d) ? d : 0.0);
+  }
 }
 
 // Plain flux q' of the face a+1/2 of cell (i, j, k) along `axis`, zero outside

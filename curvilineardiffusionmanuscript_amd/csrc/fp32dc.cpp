@@ -74,6 +74,11 @@ int cdiff_solver::ensure_fp32() {
       if ((rc = alloc32(&q32[a][b]))) return rc;
   if ((rc = alloc32(&base32))) return rc;
   if ((rc = alloc32(&a32))) return rc;
+  dc_newton = env_int("CDIFF_DC_NEWTON", 1) != 0 && kap.t_exp != 0.0 && ndim == 2;
+  if (dc_newton) {
+    if ((rc = alloc32(&ub32))) return rc;
+    if ((rc = alloc32(&da32))) return rc;
+  }
   const int ncoef = ndim == 2 ? 5 : 9;
   for (int c = 0; c < ncoef; ++c)
     if ((rc = alloc32(&coef32[c]))) return rc;
@@ -150,6 +155,10 @@ int cdiff_solver::launch_sweep32(double dt, double Vpdtau, int src, bool check) 
     A.frozen_alpha = a32;
     A.alpha_only = 1;
     A.base = base32;
+    if (dc_newton) {
+      A.ubar = ub32;
+      A.dalpha = da32;
+    }
     A.part_sum = part_sum;
     A.part_max = part_max;
     A.ctrl = ctrl_in;
@@ -197,7 +206,8 @@ int cdiff_solver::launch_sweep32(double dt, double Vpdtau, int src, bool check) 
 int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o) {
   const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
   hipLaunchKernelGGL(k_dc_update, grid_n(nfull), dim3(256), 0, stream, g, gh, kap, (const double*)rho,
-                     (const double*)u[srcbuf], e, u[dstbuf], a64, a32);
+                     (const double*)u[srcbuf], e, u[dstbuf], a64, a32, dc_newton ? ub32 : nullptr,
+                     dc_newton ? da32 : nullptr);
   LAUNCH_CHECK("k_dc_update");
   DcCoefs cf{};
   for (int t = 0; t < 9; ++t) cf.c[t] = coef[t];

@@ -63,8 +63,10 @@ void cdiff_solver::configure_sweep() {
   if (ndim == 2 && tile2d) {
     // LDS-tiled kernel: 64 x 4 cells per block
     strips = (ni + 63) / 64;
-    ch = 4;
-    chunks = (nj + 3) / 4;
+    int ty = env_int("CDIFF_TILE_TY", 8);
+    if (ty != 4 && ty != 8 && ty != 16) ty = 8;
+    ch = ty;
+    chunks = (nj + ty - 1) / ty;
     ntiles = strips * chunks;
     nblocks = (ntiles + 7) / 8 * 8;
   } else if (ndim == 2) {

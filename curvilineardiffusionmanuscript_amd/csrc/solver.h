@@ -49,6 +49,9 @@ struct cdiff_solver {
   float* coef32[9] = {nullptr};
   float *S32 = nullptr, *rho32 = nullptr, *tzA32 = nullptr, *tzB32 = nullptr;
   float* a32 = nullptr;     // alpha frozen at the FP64 iterate
+  float* ub32 = nullptr;    // FP32 copy of the FP64 iterate (Newton correction)
+  float* da32 = nullptr;    // d(alpha)/du at the FP64 iterate (Newton correction)
+  bool dc_newton = false;
   double* a64 = nullptr;    // alpha of the FP64 iterate
   cdiff::Ghost gh32{};      // ghost rules of the correction
   cdiff::Ctrl* ctrl_in = nullptr;

@@ -22,14 +22,10 @@ namespace cdiff {
 
 namespace {
 
-constexpr int TX = 64, TY = 4;
-constexpr int CX = TX + 2, CY = TY + 2;  // cells with ring
-constexpr int NCELL = CX * CY;           // 396
-constexpr int FXN = TX + 1;              // xi faces per row (x0-1 .. x0+63)
-constexpr int NFX = FXN * CY;            // 390
-constexpr int FYN = TX + 2;              // eta faces per row (x0-1 .. x0+64)
-constexpr int NFY = FYN * (TY + 1);      // 330
-constexpr int NTHR = TX * TY;            // 256
+constexpr int TX = 64;
+constexpr int CX = TX + 2;   // cells per ringed row
+constexpr int FXN = TX + 1;  // xi faces per row (x0-1 .. x0+63)
+constexpr int FYN = TX + 2;  // eta faces per row (x0-1 .. x0+64)
 
 template <typename T>
 __device__ __forceinline__ T ghost_u_t(const Sweep2DArgs<T>& A, int i, int r) {
@@ -54,6 +50,16 @@ __device__ __forceinline__ T ghost_u_t(const Sweep2DArgs<T>& A, int i, int r) {
   return m == G_NEG ? -v : v;
 }
 
+// Change of the face mean for cell changes (da0, da1) of its two conductivities:
+// harmonic 2ab/(a+b) -> (2b^2 da0 + 2a^2 da1)/(a+b)^2, arithmetic -> (da0+da1)/2.
+template <typename T>
+__device__ __forceinline__ T dmean(int harmonic, T a, T b, T da0, T da1) {
+  if (!harmonic) return T(0.5) * (da0 + da1);
+  const T s = a + b;
+  if (!(s != T(0))) return T(0);
+  return (T(2) * b * b * da0 + T(2) * a * a * da1) / (s * s);
+}
+
 template <typename T>
 __device__ __forceinline__ bool approx_eps(T x, T y) {
   if (x == y) return true;
@@ -64,10 +70,17 @@ __device__ __forceinline__ bool approx_eps(T x, T y) {
 
 }  // namespace
 
-template <typename T, bool CHECK>
-__global__ __launch_bounds__(NTHR) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
+template <typename T, bool CHECK, int TY>
+__global__ __launch_bounds__(TX* TY) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
+  constexpr int CY = TY + 2;             // ringed rows
+  constexpr int NCELL = CX * CY;         // cells with ring
+  constexpr int NFX = FXN * CY;          // xi faces
+  constexpr int NFY = FYN * (TY + 1);    // eta faces
+  constexpr int NTHR = TX * TY;
   if (A.ctrl->done) return;
   __shared__ T sU[NCELL], sA[NCELL], sT[NCELL], sD[NCELL];
+  constexpr int NNEWT = sizeof(T) == 4 ? NCELL : 1;  // the Newton correction runs in FP32 only
+  __shared__ T sUb[NNEWT], sDa[NNEWT];
   __shared__ T sX[NFX], sY[NFY];
   __shared__ T sPX[CHECK ? NFX : 1], sPY[CHECK ? NFY : 1];
   __shared__ double sRed[2][NTHR / WAVE];
@@ -92,11 +105,12 @@ __global__ __launch_bounds__(NTHR) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
   // ---- all global loads up front: one memory round trip per block
   constexpr int P1 = (NCELL + NTHR - 1) / NTHR;        // 2 cells per thread
   constexpr int P2 = (NFX + NFY + NTHR - 1) / NTHR;    // 3 faces per thread
-  T lu[P1], lr[P1], lfa[P1], lfth[P1], lfdt[P1];
+  T lu[P1], lr[P1], lfa[P1], lfth[P1], lfdt[P1], lub[P1], lda[P1];
+  const bool newton = sizeof(T) == 4 && A.ubar != nullptr;
 #pragma unroll
   for (int p = 0; p < P1; ++p) {
     const int t = tid + p * NTHR;
-    lu[p] = lr[p] = lfa[p] = lfth[p] = lfdt[p] = T(0);
+    lu[p] = lr[p] = lfa[p] = lfth[p] = lfdt[p] = lub[p] = lda[p] = T(0);
     if (t < NCELL) {
       const int i = x0 - 1 + t % CX, r = y0 - 1 + t / CX;
       if (i <= g.hi[0] + 1 && r <= g.hi[1] + 1) {
@@ -109,6 +123,10 @@ __global__ __launch_bounds__(NTHR) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
             lfth[p] = A.frozen_theta[c];
             lfdt[p] = A.frozen_dtau[c];
           }
+        }
+        if (newton) {
+          lub[p] = A.ubar[c];
+          lda[p] = A.dalpha[c];
         }
       }
     }
@@ -185,6 +203,10 @@ __global__ __launch_bounds__(NTHR) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
     sA[t] = a;
     sT[t] = th;
     sD[t] = dtau;
+    if (newton) {
+      sUb[t] = lub[p];
+      sDa[t] = lda[p];
+    }
   }
   __syncthreads();
 
@@ -202,6 +224,7 @@ __global__ __launch_bounds__(NTHR) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
         const T tbar = face_mean<T>(harm, sT[c0], sT[c1]);
         const T du = sU[c1] - sU[c0];
         qp = -abar * du;
+        if (newton) qp -= dmean<T>(harm, sA[c0], sA[c1], sDa[c0] * sU[c0], sDa[c1] * sU[c1]) * (sUb[c1] - sUb[c0]);
         qn = (lq[p] * tbar + qp) / (T(1) + tbar);
         // the xi face left of the domain has no owning cell: written here
         if (i == g.lo[0] - 1 && cy >= 1 && cy <= TY) A.qx_dst[at(g, i, r, 0)] = qn;
@@ -219,6 +242,7 @@ __global__ __launch_bounds__(NTHR) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
         const T tbar = face_mean<T>(harm, sT[c0], sT[c1]);
         const T du = sU[c1] - sU[c0];
         qp = -abar * du;
+        if (newton) qp -= dmean<T>(harm, sA[c0], sA[c1], sDa[c0] * sU[c0], sDa[c1] * sU[c1]) * (sUb[c1] - sUb[c0]);
         qn = (lq[p] * tbar + qp) / (T(1) + tbar);
         // the eta face below the domain has no owning cell: written here
         if (r == g.lo[1] - 1 && fx >= 1 && fx <= TX) A.qy_dst[at(g, i, r, 0)] = qn;
@@ -302,12 +326,21 @@ __global__ __launch_bounds__(NTHR) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
   }
 }
 
+template <typename T, int TY>
+static void launch_ty(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s) {
+  if (check)
+    hipLaunchKernelGGL((sweep2d_tile_kernel<T, true, TY>), dim3(nblocks), dim3(TX * TY), 0, s, A);
+  else
+    hipLaunchKernelGGL((sweep2d_tile_kernel<T, false, TY>), dim3(nblocks), dim3(TX * TY), 0, s, A);
+}
+
 template <typename T>
 void launch_sweep2d_tile(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s) {
-  if (check)
-    hipLaunchKernelGGL((sweep2d_tile_kernel<T, true>), dim3(nblocks), dim3(NTHR), 0, s, A);
-  else
-    hipLaunchKernelGGL((sweep2d_tile_kernel<T, false>), dim3(nblocks), dim3(NTHR), 0, s, A);
+  switch (A.ch) {  // rows per tile
+    case 4: launch_ty<T, 4>(A, check, nblocks, s); break;
+    case 16: launch_ty<T, 16>(A, check, nblocks, s); break;
+    default: launch_ty<T, 8>(A, check, nblocks, s); break;
+  }
 }
 
 template void launch_sweep2d_tile<double>(const Sweep2DArgs<double>&, bool, int, hipStream_t);

@@ -36,6 +36,11 @@ struct Sweep2DArgs {
   // the frozen alpha (alpha_only) instead of stored arrays
   const T* __restrict__ base;
   int alpha_only;
+  // Newton correction (optional): the FP64 iterate u-bar and d(alpha)/du at
+  // u-bar; the face flux of the correction then carries the linearised
+  // conductivity change, -(d abar) * delta(u-bar)
+  const T* __restrict__ ubar;
+  const T* __restrict__ dalpha;
   double* part_sum;
   double* part_max;
   const Ctrl* ctrl;
