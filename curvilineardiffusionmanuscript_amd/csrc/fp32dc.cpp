@@ -21,6 +21,8 @@
 #include <climits>
 #include <cmath>
 #include <cstdlib>
+#include <utility>
+#include <vector>
 
 #include "dc.h"
 #include "kernels.h"
@@ -124,7 +126,7 @@ int cdiff_solver::ensure_fp32() {
   return CDIFF_OK;
 }
 
-int cdiff_solver::launch_sweep32(double dt, double Vpdtau, int src, bool check) {
+int cdiff_solver::launch_sweep32(double dt, double Vpdtau, int src, bool check, bool pair) {
   if (!u32[0]) return fail(CDIFF_BAD_ARG, "FP32 buffers not set up (run an FP32 step first)");
   const int dst = 1 - src;
   const float* r32 = rho ? rho32 : nullptr;
@@ -162,7 +164,9 @@ int cdiff_solver::launch_sweep32(double dt, double Vpdtau, int src, bool check) 
     A.part_sum = part_sum;
     A.part_max = part_max;
     A.ctrl = ctrl_in;
-    if (tile2d)
+    if (pair)
+      launch_sweep2d_pair<float>(A, check, nblocks, stream);
+    else if (tile2d)
       launch_sweep2d_tile<float>(A, check, nblocks, stream);
     else
       launch_sweep2d<float>(A, check, nblocks, stream);
@@ -229,6 +233,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   const int max_outer = std::max(1, env_int("CDIFF_DC_MAX_OUTER", 100));
   const int fixed_inner = o.inner_iters;
   const int64_t total_cap = (int64_t)o.max_iter;
+  const bool pairs = pairs_ok();
   // u[0] holds T (prologue); the first FP64 residual, of T with its BCs, into u[1]
   int a = 1;
   if ((rc = dc_residual(dt, 0, nullptr, a, 1, o))) return rc;
@@ -252,18 +257,27 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     const int cap = fixed_inner > 0 ? fixed_inner : (int)std::min<int64_t>(INT_MAX - 1, std::max<int64_t>(1, total_cap - inner_total));
     int it = 0, seg = 0;
     bool pending = false;
+    int ebuf = 0;  // source buffer of the next inner launch
+    std::vector<std::pair<int, int>> ends;  // (last iteration, buffer written)
     const int seg_len = std::max(ichk, env_int("CDIFF_DC_SEG", 4 * ichk));
     while (true) {
       const int s = it + 1;
       const int e = std::min(cap, s + seg_len - 1);
-      for (int t = s; t <= e; ++t) {
-        const bool chk = fixed_inner > 0 ? (t == cap) : (t % ichk == 0 || t == cap);
-        if ((rc = launch_sweep32(dt, Vpdtau, (t - 1) & 1, chk))) return rc;
-        if (chk) {
+      for (int t = s; t <= e;) {
+        auto is_chk = [&](int x) { return fixed_inner > 0 ? (x == cap) : (x % ichk == 0 || x == cap); };
+        const bool chk = is_chk(t);
+        const bool pair = pairs && !chk && t + 1 <= e;
+        const int tl = pair ? t + 1 : t;
+        const bool chk_last = is_chk(tl);
+        if ((rc = launch_sweep32(dt, Vpdtau, ebuf, chk_last, pair))) return rc;
+        ebuf ^= 1;
+        ends.push_back({tl, ebuf});
+        if (chk_last) {
           hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(256), 0, stream, ctrl_in, (const Ctrl*)ctrl,
-                             (const double*)part_sum, nparts, N_full, fixed_inner > 0 ? 0.0 : eta, t, cap);
+                             (const double*)part_sum, nparts, N_full, fixed_inner > 0 ? 0.0 : eta, tl, cap);
           LAUNCH_CHECK("k_inner_finalize");
         }
+        t = tl + 1;
       }
       it = e;
       HIP_TRY(hipMemcpyAsync(&ctrl_h[seg & 1], ctrl_in, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
@@ -281,7 +295,10 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     const int ni = ctrl_h[3].final_iter > 0 ? ctrl_h[3].final_iter : it;
     inner_total += ni;
     inner_checks += ctrl_h[3].nchecks;
-    const float* e = u32[ni & 1];
+    int eb = ends.empty() ? 0 : ends.back().second;
+    for (const auto& pr : ends)
+      if (pr.first == ni) eb = pr.second;
+    const float* e = u32[eb];
     ++outer;
     if ((rc = dc_residual(dt, a, e, 1 - a, outer + 1, o))) return rc;
     a = 1 - a;

@@ -100,7 +100,14 @@ void cdiff_solver::configure_sweep() {
   nparts = nblocks;
 }
 
-int cdiff_solver::launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check) {
+bool cdiff_solver::pairs_ok() const {
+  if (ndim != 2 || !tile2d || ch != 8 || env_int("CDIFF_PAIRS", 1) == 0) return false;
+  for (int f = 0; f < 4; ++f)
+    if (bcs[f].kind == CDIFF_BC_PERIODIC) return false;
+  return true;
+}
+
+int cdiff_solver::launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check, bool pair) {
   const int dst = 1 - src;
   if (ndim == 2) {
     Sweep2DArgs<double> A{};
@@ -138,10 +145,14 @@ int cdiff_solver::launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, 
     A.part_sum = part_sum;
     A.part_max = part_max;
     A.ctrl = ctrl;
-    if (tile2d)
+    if (pair) {
+      A.u_pre = check ? u_pre : nullptr;
+      launch_sweep2d_pair<double>(A, check, nblocks, stream);
+    } else if (tile2d) {
       launch_sweep2d_tile<double>(A, check, nblocks, stream);
-    else
+    } else {
       launch_sweep2d<double>(A, check, nblocks, stream);
+    }
     LAUNCH_CHECK("sweep2d");
     return CDIFF_OK;
   }
@@ -207,25 +218,39 @@ int cdiff_solver::run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int 
   const int lmax = std::max(lmin, env_int("CDIFF_SEG_MAX", 256));
   const int64_t limit = (int64_t)o.max_iter + 1;
 
+  const bool pairs = pairs_ok();
+  if (pairs && !u_pre) {
+    HIP_TRY(hipMalloc((void**)&u_pre, sizeof(double) * g.total));
+    owned.push_back(u_pre);
+  }
   int64_t it = 0;
   int seg = 0;
+  int buf = cur;  // source buffer of the next launch
   bool pending = false;
+  launches.clear();
   while (true) {
     const int64_t s = it + 1;
     const int64_t cap = std::min<int64_t>(s + lmax - 1, limit);
     int64_t e = std::min<int64_t>(s + lmin - 1, cap);
     while (e < cap && !is_check((int)e, m, k)) ++e;
-    for (int64_t t = s; t <= e; ++t) {
+    for (int64_t t = s; t <= e;) {
       const bool chk = is_check((int)t, m, k);
-      const int src = (int)((cur + t - 1) & 1);
-      int rc = launch_sweep64(o, dt, Vpdtau, src, chk);
+      const bool pair = pairs && !chk && t + 1 <= e;
+      const int64_t tl = pair ? t + 1 : t;  // last iteration of this launch
+      const bool chk_last = pair ? is_check((int)tl, m, k) : chk;
+      int rc = launch_sweep64(o, dt, Vpdtau, buf, chk_last, pair);
       if (rc) return rc;
-      if (chk || t == limit) {
+      launches.push_back({(int)tl, 1 - buf, pair});
+      buf ^= 1;
+      if (chk_last || tl == limit) {
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, ctrl, (const double*)part_sum,
-                           (const double*)part_max, chk ? nparts : 0, (int)t, o.max_iter, o.rtol, o.atol, N_full,
-                           o.norm_kind, chk ? 1 : 0);
+                           (const double*)part_max, chk_last ? nparts : 0, (int)tl, o.max_iter, o.rtol, o.atol,
+                           N_full, o.norm_kind, chk_last ? 1 : 0);
         LAUNCH_CHECK("k_finalize");
+      } else if (pair && tl > limit) {
+        // cannot happen: pairs end at or before the segment end, which is capped at the limit
       }
+      t = tl + 1;
     }
     it = e;
     HIP_TRY(hipMemcpyAsync(&ctrl_h[seg & 1], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
@@ -392,6 +417,17 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   }
   const int fi = c.final_iter;
   int src = (int)((cur + fi - 1) & 1), dst = 1 - src;
+  const double* pre = nullptr;
+  if (!dc) {
+    // the launch that ended at the final iteration (checks always end a launch)
+    for (const auto& L : launches)
+      if (L.last_iter == fi) {
+        dst = L.dst;
+        src = 1 - dst;
+        pre = L.pair ? u_pre : u[src];
+      }
+    if (!pre) pre = u[src];
+  }
   last_iter = fi;
   if (dc) {
     // final iterate (with the BC ghosts of its own domain values) in u[dst];
@@ -408,8 +444,8 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
                          (const double*)u[dst], (const double*)a64, q[ax][dst]);
     LAUNCH_CHECK("dc epilogue");
   } else if (o.subcycle_conductivity) {
-    hipLaunchKernelGGL(k_alpha_full, grid_full(g), dim3(256), 0, stream, g, kap, (const double*)rho,
-                       (const double*)u[src], (const double*)u[dst], alpha);
+    hipLaunchKernelGGL(k_alpha_full, grid_full(g), dim3(256), 0, stream, g, kap, (const double*)rho, pre,
+                       (const double*)u[dst], alpha);
   } else {
     hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)frozen[0],
                        alpha);
@@ -426,6 +462,7 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   if (next_dt_out) *next_dt_out = o.calculate_next_dt ? red_h[0] : dt;
   cur = dst;
   last_src = src;
+  last_pre = dc ? u[dst] : pre;
   have_last = true;
   t_written = true;
   const auto t1 = std::chrono::steady_clock::now();
@@ -512,8 +549,8 @@ int cdiff_solver::get_field(int field, double* out) {
     release();
     return fail(CDIFF_HIP_ERROR, "hipMalloc(scratch)");
   }
-  hipLaunchKernelGGL(k_merge, grid_full(g), dim3(256), 0, stream, g, (const double*)u[src], (const double*)u[dst],
-                     merged);
+  hipLaunchKernelGGL(k_merge, grid_full(g), dim3(256), 0, stream, g, last_pre ? last_pre : (const double*)u[src],
+                     (const double*)u[dst], merged);
   if (last_subcycle)
     hipLaunchKernelGGL(k_alpha_full, grid_full(g), dim3(256), 0, stream, g, kap, (const double*)rho,
                        (const double*)nullptr, (const double*)merged, a_pre);
@@ -553,14 +590,18 @@ int cdiff_solver::bench_sweeps(double dt, int precision, int check, int n, doubl
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
+  const bool pair = (check & 2) != 0;
+  if (pair && (!pairs_ok() || (n & 1)))
+    return fail(CDIFF_BAD_ARG, "two-iteration launches need a 2D mesh without periodic faces and an even nsweeps");
   int rc = CDIFF_OK;
+  int nl = 0;
   (void)hipEventRecord(e0, stream);
-  for (int t = 0; t < n && rc == CDIFF_OK; ++t) {
-    const int src = (cur + t) & 1;
+  for (int t = 0; t < n && rc == CDIFF_OK; t += pair ? 2 : 1, ++nl) {
+    const int src = (cur + nl) & 1;
     if (precision == CDIFF_PREC_FP64)
-      rc = launch_sweep64(o, dt, Vpdtau, src, check != 0);
+      rc = launch_sweep64(o, dt, Vpdtau, src, (check & 1) != 0, pair);
     else
-      rc = launch_sweep32(dt, Vpdtau, src, check != 0);
+      rc = launch_sweep32(dt, Vpdtau, src, (check & 1) != 0, pair);
   }
   (void)hipEventRecord(e1, stream);
   hipError_t se = hipEventSynchronize(e1);
@@ -570,7 +611,7 @@ int cdiff_solver::bench_sweeps(double dt, int precision, int check, int n, doubl
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   if (se != hipSuccess) return hip_fail(se, "hipEventSynchronize");
-  cur = (cur + n) & 1;
+  cur = (cur + nl) & 1;
   have_last = false;
   *mean_ms = ms / n;
   return CDIFF_OK;

@@ -71,12 +71,21 @@ struct cdiff_solver {
   int cur = 0;        // ping-pong index of the current q (and u) buffers
   int last_iter = 0;  // solver.last_err_check
   bool have_last = false;
-  int last_src = 0;  // buffer holding the pre-update iterate of the last iteration
+  int last_src = 0;  // buffer written by the last launch's source (pre-update iterate, single launches)
+  double* u_pre = nullptr;       // pre-update iterate of a two-iteration launch ending in a check
+  const double* last_pre = nullptr;  // domain values of the iterate before the last iteration
   double last_dt = 0;
   double last_Vpdtau = 0;
   int last_subcycle = 1;
   int enforce_pos_last = 0;
   bool t_written = false;
+
+  struct Launch {
+    int last_iter;  // last pseudo-iteration of the launch
+    int dst;        // buffer it wrote
+    bool pair;      // two iterations
+  };
+  std::vector<Launch> launches;  // of the current step
 
   std::string err;
 
@@ -89,8 +98,9 @@ struct cdiff_solver {
   int get_field(int field, double* out);
   int validation_message(int status);
   int step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cdiff_stats* stats);
-  int launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check);
-  int launch_sweep32(double dt, double Vpdtau, int src, bool check);
+  int launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check, bool pair = false);
+  int launch_sweep32(double dt, double Vpdtau, int src, bool check, bool pair = false);
+  bool pairs_ok() const;
   int bench_sweeps(double dt, int precision, int check, int n, double* mean_ms);
   int ensure_fp32();
   int dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o);

@@ -41,6 +41,9 @@ struct Sweep2DArgs {
   // conductivity change, -(d abar) * delta(u-bar)
   const T* __restrict__ ubar;
   const T* __restrict__ dalpha;
+  // two-iteration launches with a check: the iterate before the last
+  // iteration (domain cells), for the step epilogue (nullable)
+  T* __restrict__ u_pre;
   double* part_sum;
   double* part_max;
   const Ctrl* ctrl;
@@ -87,6 +90,9 @@ struct Sweep3DArgs {
 
 template <typename T>
 void launch_sweep3d(const Sweep3DArgs<T>& A, bool check, int nblocks, hipStream_t s);
+
+template <typename T>
+void launch_sweep2d_pair(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s);
 
 template <typename T>
 void launch_sweep2d_tile(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s);

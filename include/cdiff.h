@@ -214,9 +214,10 @@ int cdiff_reset_state(cdiff_solver *h);
 /* Measurement helper (no counterpart in the reference): run `nsweeps`
  * back-to-back fused sweeps of the current state on the handle's stream
  * (FP64, or the FP32 correction sweep with precision = CDIFF_PREC_FP32_DC),
- * with (check = 1) or without the residual, bracketed by HIP events; returns
- * the mean kernel time in milliseconds.  Clobbers the iterate: call it after
- * the steps being measured. */
+ * with (check bit 0 set) or without the residual, bracketed by HIP events;
+ * returns the mean time per pseudo-iteration in milliseconds.  With check bit
+ * 1 set, the iterations run two per launch (2D without periodic faces; even
+ * nsweeps).  Clobbers the iterate: call it after the steps being measured. */
 int cdiff_bench_sweeps(cdiff_solver *h, double dt, int precision, int check, int nsweeps, double *mean_ms);
 
 const char *cdiff_last_error(const cdiff_solver *h);
