@@ -306,6 +306,7 @@ int cdiff_set_metrics(cdiff_solver* h, const double* const* cell, const double* 
   for (int a = 0; a < nd; ++a) h->spacing[a] = spacing[a];
   h->metrics_set = true;
   h->coefs32_valid = false;
+  h->lin_metrics_valid = false;
   return CDIFF_OK;
 }
 

@@ -16,6 +16,15 @@ __global__ void k_dc_residual(Geo g, int harmonic, double dt, const double* u, c
 __global__ void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* psum, int nparts, double nfull, double eta,
                                  int iter, int cap);
 __global__ void k_fill32(int64_t n, float v, float* dst);
+__global__ void k_lin_metrics(Geo g, const double* g11, const double* g22, const double* c12, const double* ac,
+                              const double* bc, float* m0, float* m1, float* m2, float* m3, float* m4);
+struct LinOut {
+  float* fx[3];
+  float* fy[3];
+  float* cc;
+};
+__global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double t_exp, int newton, const double* rho,
+                            const double* u, const double* a, LinOut out);
 __global__ void k_to_f32(int64_t n, const double* src, float* dst);
 
 }  // namespace cdiff

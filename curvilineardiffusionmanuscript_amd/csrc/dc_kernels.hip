@@ -224,4 +224,111 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ src, float* __res
   if (t < n) dst[t] = (float)src[t];
 }
 
+// ---------------------------------------------------------------------------
+// Coefficients of the FP32 correction sweep in coefficient form (2D).
+
+// Divergence coefficients folded per cell (constant per mesh):
+// D = m0 x_c + m1 x_w + m2 y_c + m3 y_s + m4 (cross terms).
+__global__ void k_lin_metrics(Geo g, const double* __restrict__ g11, const double* __restrict__ g22,
+                              const double* __restrict__ c12, const double* __restrict__ ac,
+                              const double* __restrict__ bc, float* __restrict__ m0, float* __restrict__ m1,
+                              float* __restrict__ m2, float* __restrict__ m3, float* __restrict__ m4) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t ntot = (int64_t)g.n[0] * g.n[1];
+  if (t >= ntot) return;
+  const int i = (int)(t % g.n[0]), j = (int)(t / g.n[0]);
+  const int64_t c = at(g, i, j, 0);
+  const bool dom = i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1];
+  if (!dom) {
+    m0[c] = m1[c] = m2[c] = m3[c] = m4[c] = 0.f;
+    return;
+  }
+  const double ha = 0.5 * ac[c], hb = 0.5 * bc[c];
+  m0[c] = (float)(g11[c] + ha);
+  m1[c] = (float)(ha - g11[c]);
+  m2[c] = (float)(g22[c] + hb);
+  m3[c] = (float)(hb - g22[c]);
+  m4[c] = (float)c12[c];
+}
+
+namespace {
+
+// factor s of the correction in the first ghost layer of face f: e_ghost = s e_source
+__device__ __forceinline__ double ghost_factor(const Ghost& gh, int f) {
+  const int m = gh.mode[f];
+  return m == G_COPY ? 1.0 : (m == G_NEG ? -1.0 : 0.0);
+}
+
+// d(alpha)/du of kappa0 |rho|^a |u|^b / (rho cp): b alpha / u (0 where undefined)
+__device__ __forceinline__ double dalpha_du(double t_exp, double a, double u) {
+  const double d = (u != 0.0 && isfinite(u)) ? t_exp * a / u : 0.0;
+  return isfinite(d) ? d : 0.0;
+}
+
+}  // namespace
+
+// Per face of expand_lower(domain, axis), from the FP64 iterate u and its
+// alpha a (full arrays, ghosts by the BC rules): P, M, W of
+//   q' = P e0 - M e1,  q <- q' + W (q - q')
+// the linearisation of q'(u + e) - q'(u) in e, with the Newton term of the
+// face mean when `newton`; per domain cell C = dtau / (1 + dtau/dt).
+// Ghost cells of boundary faces are folded: e_ghost = s e_domain.
+__global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double t_exp, int newton,
+                            const double* __restrict__ rho, const double* __restrict__ u,
+                            const double* __restrict__ a, LinOut out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t ntot = (int64_t)g.n[0] * g.n[1];
+  if (t >= ntot) return;
+  const int i = (int)(t % g.n[0]), j = (int)(t / g.n[0]);
+  const int64_t c = at(g, i, j, 0);
+  const bool dom = i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1];
+  auto params = [&](int64_t cell, double& dtau, double& theta) {
+    iter_params<double>(ic, rho ? rho[cell] : 1.0, a[cell], dtau, theta);
+  };
+  double dtau0 = 0.0, th0 = 0.0;
+  const bool need0 = i >= g.lo[0] - 1 && i <= g.hi[0] + 1 && j >= g.lo[1] - 1 && j <= g.hi[1] + 1;
+  if (need0) params(c, dtau0, th0);
+  out.cc[c] = dom ? (float)(dtau0 / (1.0 + dtau0 / ic.dt)) : 0.f;
+  for (int ax = 0; ax < 2; ++ax) {
+    float* const* F = ax == 0 ? out.fx : out.fy;
+    const bool on = ax == 0 ? (i >= g.lo[0] - 1 && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1])
+                            : (i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] - 1 && j <= g.hi[1]);
+    if (!on) {
+      F[0][c] = F[1][c] = F[2][c] = 0.f;
+      continue;
+    }
+    const int64_t c1 = ax == 0 ? at(g, i + 1, j, 0) : at(g, i, j + 1, 0);
+    double dtau1, th1;
+    params(c1, dtau1, th1);
+    const double a0 = a[c], a1 = a[c1];
+    const double abar = face_mean<double>(harmonic, a0, a1);
+    const double tbar = face_mean<double>(harmonic, th0, th1);
+    double P = abar, M = abar;
+    if (newton) {
+      const double s = a0 + a1;
+      double d0 = 0.5, d1 = 0.5;  // derivatives of the face mean in a0, a1
+      if (harmonic) {
+        d0 = (s != 0.0) ? 2.0 * a1 * a1 / (s * s) : 0.0;
+        d1 = (s != 0.0) ? 2.0 * a0 * a0 / (s * s) : 0.0;
+      }
+      const double du = u[c1] - u[c];
+      P -= d0 * dalpha_du(t_exp, a0, u[c]) * du;
+      M += d1 * dalpha_du(t_exp, a1, u[c1]) * du;
+    }
+    // boundary faces: cell 0 below the domain, or cell 1 above it
+    const int lo = ax == 0 ? g.lo[0] : g.lo[1], hi = ax == 0 ? g.hi[0] : g.hi[1];
+    const int idx = ax == 0 ? i : j;
+    if (idx == lo - 1) {
+      M -= ghost_factor(gh, 2 * ax) * P;
+      P = 0.0;
+    } else if (idx == hi) {
+      P -= ghost_factor(gh, 2 * ax + 1) * M;
+      M = 0.0;
+    }
+    F[0][c] = (float)P;
+    F[1][c] = (float)M;
+    F[2][c] = (float)(tbar / (1.0 + tbar));
+  }
+}
+
 }  // namespace cdiff

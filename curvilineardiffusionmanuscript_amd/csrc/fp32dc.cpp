@@ -114,6 +114,41 @@ int cdiff_solver::ensure_fp32() {
     LAUNCH_CHECK("k_to_f32");
   }
   aux32_valid = true;
+  // coefficient-form sweep: 2D without periodic faces
+  lin = ndim == 2 && env_int("CDIFF_DC_LIN", 1) != 0;
+  for (int f = 0; f < 2 * ndim; ++f)
+    if (bcs[f].kind == CDIFF_BC_PERIODIC) lin = false;
+  if (lin) {
+    lin_k = env_int("CDIFF_LIN_K", 4);
+    if (lin_k != 2 && lin_k != 4 && lin_k != 6) lin_k = 4;
+    const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
+    const int rows = lin2d_rows(lin_k);
+    lin_chunks = (nj + rows - 1) / rows;
+    lin_tiles = ((ni + 63) / 64) * lin_chunks;
+    lin_nblocks = (lin_tiles + 7) / 8 * 8;
+    for (int c = 0; c < 5; ++c)
+      if ((rc = alloc32(&lin_m[c]))) return rc;
+    for (int c = 0; c < 3; ++c) {
+      if ((rc = alloc32(&lin_fx[c]))) return rc;
+      if ((rc = alloc32(&lin_fy[c]))) return rc;
+    }
+    if ((rc = alloc32(&lin_cc))) return rc;
+    if (lin_nblocks > lin_nparts) {
+      HIP_TRY(hipMalloc((void**)&lin_psum, sizeof(double) * lin_nblocks));
+      owned.push_back(lin_psum);
+      HIP_TRY(hipMalloc((void**)&lin_pmax, sizeof(double) * lin_nblocks));
+      owned.push_back(lin_pmax);
+      lin_nparts = lin_nblocks;
+    }
+    if (!lin_metrics_valid) {
+      hipLaunchKernelGGL(k_lin_metrics, grid_n((int64_t)g.n[0] * g.n[1]), dim3(256), 0, stream, g,
+                         (const double*)coef[0], (const double*)coef[1], (const double*)coef[2],
+                         (const double*)coef[3], (const double*)coef[4], lin_m[0], lin_m[1], lin_m[2], lin_m[3],
+                         lin_m[4]);
+      LAUNCH_CHECK("k_lin_metrics");
+      lin_metrics_valid = true;
+    }
+  }
   // ghost rules of the correction: Dirichlet (and untouched) ghosts hold no
   // correction; copy / periodic / negated copies carry over unchanged
   gh32 = gh;
@@ -206,6 +241,54 @@ int cdiff_solver::launch_sweep32(double dt, double Vpdtau, int src, bool check, 
   return CDIFF_OK;
 }
 
+// Coefficients of the correction sweep at the FP64 iterate u[buf] (alpha in a64).
+int cdiff_solver::lin_coefs(double dt, double Vpdtau, int buf) {
+  LinOut out{};
+  for (int c = 0; c < 3; ++c) {
+    out.fx[c] = lin_fx[c];
+    out.fy[c] = lin_fy[c];
+  }
+  out.cc = lin_cc;
+  IterConst ic{L, Vpdtau, dt};
+  hipLaunchKernelGGL(k_lin_coefs, grid_n((int64_t)g.n[0] * g.n[1]), dim3(256), 0, stream, g, gh,
+                     (int)(mean == CDIFF_MEAN_HARMONIC), ic, kap.t_exp, (int)dc_newton, (const double*)rho,
+                     (const double*)u[buf], (const double*)a64, out);
+  LAUNCH_CHECK("k_lin_coefs");
+  lin_coefs_ready = true;
+  return CDIFF_OK;
+}
+
+int cdiff_solver::launch_lin(double dt, int src, bool check, int nit) {
+  if (!lin || !lin_coefs_ready) return fail(CDIFF_BAD_ARG, "coefficient-form sweep not set up (run an FP32 step first)");
+  if (nit < 1 || nit > lin_k) return fail(CDIFF_BAD_ARG, "iterations per launch out of range");
+  const int dst = 1 - src;
+  Lin2DArgs A{};
+  A.g = g;
+  A.ntiles = lin_tiles;
+  A.chunks = lin_chunks;
+  A.nit = nit;
+  A.inv_dt = (float)(1.0 / dt);
+  A.e_src = u32[src];
+  A.e_dst = u32[dst];
+  A.qx_src = q32[0][src];
+  A.qx_dst = q32[0][dst];
+  A.qy_src = q32[1][src];
+  A.qy_dst = q32[1][dst];
+  A.base = base32;
+  A.cc = lin_cc;
+  for (int c = 0; c < 5; ++c) A.m[c] = lin_m[c];
+  for (int c = 0; c < 3; ++c) {
+    A.fx[c] = lin_fx[c];
+    A.fy[c] = lin_fy[c];
+  }
+  A.part_sum = lin_psum;
+  A.part_max = lin_pmax;
+  A.ctrl = ctrl_in;
+  launch_sweep2d_lin(A, lin_k, check, lin_nblocks, stream);
+  LAUNCH_CHECK("sweep2d_lin");
+  return CDIFF_OK;
+}
+
 // residual of the iterate in u[dstbuf] after u[dstbuf] = u[srcbuf] + e
 int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o) {
   const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
@@ -234,6 +317,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   const int fixed_inner = o.inner_iters;
   const int64_t total_cap = (int64_t)o.max_iter;
   const bool pairs = pairs_ok();
+  const int lin_nit = std::max(1, std::min(lin_k, env_int("CDIFF_LIN_NIT", lin_k)));
   // u[0] holds T (prologue); the first FP64 residual, of T with its BCs, into u[1]
   int a = 1;
   if ((rc = dc_residual(dt, 0, nullptr, a, 1, o))) return rc;
@@ -254,6 +338,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
       for (int b = 0; b < 2; ++b) hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, q32[ax][b]);
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl_in, 0.0, 0.0);
     LAUNCH_CHECK("inner reset");
+    if (lin && (rc = lin_coefs(dt, Vpdtau, a))) return rc;
     const int cap = fixed_inner > 0 ? fixed_inner : (int)std::min<int64_t>(INT_MAX - 1, std::max<int64_t>(1, total_cap - inner_total));
     int it = 0, seg = 0;
     bool pending = false;
@@ -265,16 +350,23 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
       const int e = std::min(cap, s + seg_len - 1);
       for (int t = s; t <= e;) {
         auto is_chk = [&](int x) { return fixed_inner > 0 ? (x == cap) : (x % ichk == 0 || x == cap); };
-        const bool chk = is_chk(t);
-        const bool pair = pairs && !chk && t + 1 <= e;
-        const int tl = pair ? t + 1 : t;
+        int tl = t;
+        if (lin) {
+          // up to lin_nit iterations; a check iteration ends the launch
+          while (tl < e && tl - t + 1 < lin_nit && !is_chk(tl)) ++tl;
+          if ((rc = launch_lin(dt, ebuf, is_chk(tl), tl - t + 1))) return rc;
+        } else {
+          const bool pair = pairs && !is_chk(t) && t + 1 <= e;
+          tl = pair ? t + 1 : t;
+          if ((rc = launch_sweep32(dt, Vpdtau, ebuf, is_chk(tl), pair))) return rc;
+        }
         const bool chk_last = is_chk(tl);
-        if ((rc = launch_sweep32(dt, Vpdtau, ebuf, chk_last, pair))) return rc;
         ebuf ^= 1;
         ends.push_back({tl, ebuf});
         if (chk_last) {
           hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(256), 0, stream, ctrl_in, (const Ctrl*)ctrl,
-                             (const double*)part_sum, nparts, N_full, fixed_inner > 0 ? 0.0 : eta, tl, cap);
+                             (const double*)(lin ? lin_psum : part_sum), lin ? lin_nblocks : nparts, N_full,
+                             fixed_inner > 0 ? 0.0 : eta, tl, cap);
           LAUNCH_CHECK("k_inner_finalize");
         }
         t = tl + 1;

@@ -591,14 +591,20 @@ int cdiff_solver::bench_sweeps(double dt, int precision, int check, int n, doubl
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   const bool pair = (check & 2) != 0;
+  const bool lin_sweep = (check & 4) != 0 && precision != CDIFF_PREC_FP64;
   if (pair && (!pairs_ok() || (n & 1)))
     return fail(CDIFF_BAD_ARG, "two-iteration launches need a 2D mesh without periodic faces and an even nsweeps");
+  if (lin_sweep && (!lin || !lin_coefs_ready || n % lin_k))
+    return fail(CDIFF_BAD_ARG, "coefficient-form sweeps need a prior FP32 step and nsweeps a multiple of their depth");
   int rc = CDIFF_OK;
   int nl = 0;
+  const int per = lin_sweep ? lin_k : (pair ? 2 : 1);
   (void)hipEventRecord(e0, stream);
-  for (int t = 0; t < n && rc == CDIFF_OK; t += pair ? 2 : 1, ++nl) {
+  for (int t = 0; t < n && rc == CDIFF_OK; t += per, ++nl) {
     const int src = (cur + nl) & 1;
-    if (precision == CDIFF_PREC_FP64)
+    if (lin_sweep)
+      rc = launch_lin(dt, nl & 1, (check & 1) != 0, lin_k);
+    else if (precision == CDIFF_PREC_FP64)
       rc = launch_sweep64(o, dt, Vpdtau, src, (check & 1) != 0, pair);
     else
       rc = launch_sweep32(dt, Vpdtau, src, (check & 1) != 0, pair);

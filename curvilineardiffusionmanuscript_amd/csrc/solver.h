@@ -62,6 +62,20 @@ struct cdiff_solver {
   std::vector<void*> owned32;
   bool coefs32_valid = false, aux32_valid = false, q32_valid = false;
 
+  // FP32 correction sweep in coefficient form (2D, no periodic faces)
+  bool lin = false;
+  int lin_k = 4;  // iterations per launch at most
+  int lin_tiles = 0, lin_chunks = 0, lin_nblocks = 0;
+  float* lin_m[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  float* lin_fx[3] = {nullptr, nullptr, nullptr};
+  float* lin_fy[3] = {nullptr, nullptr, nullptr};
+  float* lin_cc = nullptr;
+  bool lin_metrics_valid = false;
+  bool lin_coefs_ready = false;
+  double* lin_psum = nullptr;
+  double* lin_pmax = nullptr;
+  int lin_nparts = 0;
+
   // sweep decomposition
   bool tile2d = true;  // 2D: LDS-tiled kernel (else the row-marching kernel)
   int strips = 0, chunks = 0, ch = 0, ntiles = 0, nblocks = 0;
@@ -100,6 +114,8 @@ struct cdiff_solver {
   int step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cdiff_stats* stats);
   int launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check, bool pair = false);
   int launch_sweep32(double dt, double Vpdtau, int src, bool check, bool pair = false);
+  int launch_lin(double dt, int src, bool check, int nit);
+  int lin_coefs(double dt, double Vpdtau, int buf);
   bool pairs_ok() const;
   int bench_sweeps(double dt, int precision, int check, int n, double* mean_ms);
   int ensure_fp32();

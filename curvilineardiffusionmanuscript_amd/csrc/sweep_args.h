@@ -100,4 +100,30 @@ void launch_sweep2d_tile(const Sweep2DArgs<T>& A, bool check, int nblocks, hipSt
 template <typename T>
 void launch_sweep2d(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s);
 
+// FP32 correction sweep in coefficient form (sweep2d_lin.hip).
+struct Lin2DArgs {
+  Geo g;
+  int ntiles, chunks;  // tiles of 64 x lin2d_rows(kmax) cells; chunks along j
+  int nit;             // iterations of this launch, 1..kmax
+  float inv_dt;
+  const float* __restrict__ e_src;
+  float* __restrict__ e_dst;
+  const float* __restrict__ qx_src;
+  float* __restrict__ qx_dst;
+  const float* __restrict__ qy_src;
+  float* __restrict__ qy_dst;
+  const float* __restrict__ base;  // right-hand side (FP64 residual)
+  const float* __restrict__ cc;    // dtau / (1 + dtau/dt) per domain cell
+  const float* m[5];               // g11 + ac/2, ac/2 - g11, g22 + bc/2, bc/2 - g22, c12
+  const float* fx[3];              // P, M, W of the xi faces
+  const float* fy[3];              // P, M, W of the eta faces
+  double* part_sum;
+  double* part_max;
+  const Ctrl* ctrl;
+};
+
+// kmax: 2, 4 or 6 iterations per launch at most
+int lin2d_rows(int kmax);
+void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, bool check, int nblocks, hipStream_t s);
+
 }  // namespace cdiff

@@ -217,7 +217,10 @@ int cdiff_reset_state(cdiff_solver *h);
  * with (check bit 0 set) or without the residual, bracketed by HIP events;
  * returns the mean time per pseudo-iteration in milliseconds.  With check bit
  * 1 set, the iterations run two per launch (2D without periodic faces; even
- * nsweeps).  Clobbers the iterate: call it after the steps being measured. */
+ * nsweeps); with bit 2 and precision = CDIFF_PREC_FP32_DC, the coefficient-form
+ * correction sweep runs at its full depth per launch (after an FP32 step;
+ * nsweeps a multiple of the depth).  Clobbers the iterate: call it after the
+ * steps being measured. */
 int cdiff_bench_sweeps(cdiff_solver *h, double dt, int precision, int check, int nsweeps, double *mean_ms);
 
 const char *cdiff_last_error(const cdiff_solver *h);

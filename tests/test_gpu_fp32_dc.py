@@ -32,8 +32,14 @@ def _oracle_residual_rms(mesh, bcs_o, kappa, T_new, T_old, dt):
     return o.l2_norm(o.res, 0)
 
 
-@pytest.mark.parametrize("case", ["config1", "config2", "config4"])
-def test_dc_reaches_fp64_tolerance(case):
+@pytest.mark.parametrize(
+    "case,lin",
+    [("config1", "1"), ("config2", "1"), ("config1", "0"), ("config2", "0"), ("config4", "1")],
+)
+def test_dc_reaches_fp64_tolerance(case, lin, monkeypatch):
+    # lin = 1: the 2D correction sweep in coefficient form (several iterations
+    # per launch); lin = 0: the conductivity-form sweep
+    monkeypatch.setenv("CDIFF_DC_LIN", lin)
     atol = 1e-8
     if case == "config1":
         mesh, T0 = gaussian_2d(96, ni=100)
@@ -80,3 +86,30 @@ def test_dc_then_fp64_step_continues():
     st, _ = S.nonlinear_thermal_conduction_step(dev, mesh, T, 1.0, 1.0, law, 1e-4, rtol=0.0, atol=1e-8,
                                                 max_iter=10**6, show_convergence=False)
     assert st.abs_err <= 1e-8 and st.niter > 0
+
+
+@pytest.mark.parametrize("depth", ["2", "4", "6"])
+def test_lin_sweep_depth_does_not_change_result(depth, monkeypatch):
+    # the coefficient-form sweep runs up to `depth` iterations per launch on a
+    # tile with a depth-cell ring; per-cell arithmetic does not depend on how
+    # the iterations are grouped, so one iteration per launch gives the same bits
+    mesh, T0 = hot_disk_2d(150, seed=4)
+    bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.FixedNegSymmetryBC(), S.NeumannBC()]
+    law = S.PowerLawConductivity(1.0, 2.5, 0.0)
+    monkeypatch.setenv("CDIFF_LIN_K", depth)
+    out = []
+    for nit in (depth, "1"):
+        monkeypatch.setenv("CDIFF_LIN_NIT", nit)
+        dev = S.PseudoTransientSolver(mesh, bcs)
+        T = T0.copy(order="F")
+        S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
+        its = []
+        for _ in range(2):
+            st, _ = S.nonlinear_thermal_conduction_step(dev, mesh, T, 1.0, 1.0, law, 1e-5, rtol=0.0, atol=1e-8,
+                                                        max_iter=10**6, precision="fp32_dc",
+                                                        calculate_next_dt=False, show_convergence=False)
+            its.append((st.niter, st.outer, st.abs_err))
+        dev.close()
+        out.append((its, T))
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])

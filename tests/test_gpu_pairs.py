@@ -112,6 +112,7 @@ def test_pairs_bit_identical_forcing():
 @pytest.mark.parametrize("newton", ["0", "1"])
 def test_pairs_bit_identical_fp32_dc(newton, monkeypatch):
     monkeypatch.setenv("CDIFF_DC_NEWTON", newton)
+    monkeypatch.setenv("CDIFF_DC_LIN", "0")  # the conductivity-form FP32 sweep
     mesh, T0 = hot_disk_2d(96, seed=0)
     bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
     _same(mesh, T0, bcs, S.PowerLawConductivity(1.0, 2.5, 0.0), 3, 1e-5, fp32=True, rtol=0.0, atol=1e-8,
