@@ -23,6 +23,32 @@ struct LinOut {
   float* fy[3];
   float* cc;
 };
+// Fused outer pass of the 2D step (dc_outer2d.hip): u_new = u_old + e with
+// local BC rules (no periodic faces), alpha, the FP64 residual (into r32 and
+// block partials over 64 x 8 tiles), and the correction-sweep coefficients.
+struct DcOuter2D {
+  Geo g;
+  Ghost gh;
+  KappaLaw kap;
+  IterConst ic;
+  int harmonic, newton, ntiles, chunks;
+  double t_exp;
+  const double* rho;
+  const double* u_old;
+  const float* e;  // nullable: no correction
+  double* u_new;
+  double* a64;
+  const double* uprev;
+  const double* c[5];  // g11 g22 c12 ac bc
+  const double *S, *tzA, *tzB;
+  float* r32;
+  float* fx[3];
+  float* fy[3];
+  float* cc;
+  double* psum;
+  double* pmax;
+};
+__global__ void k_dc_outer2d(DcOuter2D A);
 __global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double t_exp, int newton, const double* rho,
                             const double* u, const double* a, LinOut out);
 __global__ void k_to_f32(int64_t n, const double* src, float* dst);

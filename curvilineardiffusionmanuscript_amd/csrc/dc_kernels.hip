@@ -191,15 +191,27 @@ __global__ __launch_bounds__(256) void k_dc_residual(Geo g, int harmonic, double
 
 // Inner-loop decision: stop the FP32 sweeps once the correction residual has
 // dropped below eta * (the outer FP64 residual), or after `cap` sweeps.
-__global__ __launch_bounds__(256) void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* __restrict__ psum,
-                                                        int nparts, double nfull, double eta, int iter, int cap) {
+__global__ __launch_bounds__(1024) void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* __restrict__ psum,
+                                                         int nparts, double nfull, double eta, int iter, int cap) {
   if (cin->done) return;
-  __shared__ double ss[256];
+  constexpr int NT = 1024, U = 8;
+  __shared__ double ss[NT];
+  double as[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) as[u] = 0.0;
+  for (int base = 0; base < nparts; base += NT * U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = base + u * NT + threadIdx.x;
+      if (p < nparts) as[u] += psum[p];
+    }
+  }
   double s = 0.0;
-  for (int p = threadIdx.x; p < nparts; p += 256) s += psum[p];
+#pragma unroll
+  for (int u = 0; u < U; ++u) s += as[u];
   ss[threadIdx.x] = s;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = NT / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) ss[threadIdx.x] += ss[threadIdx.x + o];
     __syncthreads();
   }

@@ -292,6 +292,53 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit) {
 // residual of the iterate in u[dstbuf] after u[dstbuf] = u[srcbuf] + e
 int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o) {
   const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  if (lin) {
+    // one fused pass: update, FP64 residual, coefficients of the next sweep
+    if (!e) {
+      // alpha over the full array once (corners and deeper halo layers keep it)
+      hipLaunchKernelGGL(k_alpha_full, grid_n(nfull), dim3(256), 0, stream, g, kap, (const double*)rho,
+                         (const double*)nullptr, (const double*)u[srcbuf], a64);
+      LAUNCH_CHECK("k_alpha_full");
+    }
+    const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
+    DcOuter2D A{};
+    A.g = g;
+    A.gh = gh;
+    A.kap = kap;
+    A.ic = IterConst{L, dc_Vpdtau, dt};
+    A.harmonic = mean == CDIFF_MEAN_HARMONIC;
+    A.newton = dc_newton;
+    A.chunks = (nj + 7) / 8;
+    A.ntiles = ((ni + 63) / 64) * A.chunks;
+    A.t_exp = kap.t_exp;
+    A.rho = rho;
+    A.u_old = u[srcbuf];
+    A.e = e;
+    A.u_new = u[dstbuf];
+    A.a64 = a64;
+    A.uprev = u_prev;
+    for (int c = 0; c < 5; ++c) A.c[c] = coef[c];
+    A.S = S;
+    A.tzA = tzA;
+    A.tzB = tzB;
+    A.r32 = base32;
+    for (int c = 0; c < 3; ++c) {
+      A.fx[c] = lin_fx[c];
+      A.fy[c] = lin_fy[c];
+    }
+    A.cc = lin_cc;
+    A.psum = dc_psum;
+    A.pmax = dc_pmax;
+    const int nblk = (A.ntiles + 7) / 8 * 8;
+    if (nblk > dc_nparts) return fail(CDIFF_HIP_ERROR, "partial-sum buffer too small");
+    hipLaunchKernelGGL(k_dc_outer2d, dim3(nblk), dim3(512), 0, stream, A);
+    LAUNCH_CHECK("k_dc_outer2d");
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl, (const double*)dc_psum,
+                       (const double*)dc_pmax, nblk, iter, INT_MAX, o.rtol, o.atol, N_full, o.norm_kind, 1);
+    LAUNCH_CHECK("k_finalize");
+    lin_coefs_ready = true;
+    return CDIFF_OK;
+  }
   hipLaunchKernelGGL(k_dc_update, grid_n(nfull), dim3(256), 0, stream, g, gh, kap, (const double*)rho,
                      (const double*)u[srcbuf], e, u[dstbuf], a64, a32, dc_newton ? ub32 : nullptr,
                      dc_newton ? da32 : nullptr);
@@ -302,7 +349,7 @@ int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf,
                      (const double*)u[dstbuf], (const double*)a64, (const double*)u_prev, cf, (const double*)S,
                      (const double*)tzA, (const double*)tzB, base32, dc_psum, dc_pmax);
   LAUNCH_CHECK("k_dc_residual");
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, ctrl, (const double*)dc_psum, (const double*)dc_pmax,
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl, (const double*)dc_psum, (const double*)dc_pmax,
                      (int)((nfull + 255) / 256), iter, INT_MAX, o.rtol, o.atol, N_full, o.norm_kind, 1);
   LAUNCH_CHECK("k_finalize");
   return CDIFF_OK;
@@ -318,6 +365,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   const int64_t total_cap = (int64_t)o.max_iter;
   const bool pairs = pairs_ok();
   const int lin_nit = std::max(1, std::min(lin_k, env_int("CDIFF_LIN_NIT", lin_k)));
+  dc_Vpdtau = Vpdtau;
   // u[0] holds T (prologue); the first FP64 residual, of T with its BCs, into u[1]
   int a = 1;
   if ((rc = dc_residual(dt, 0, nullptr, a, 1, o))) return rc;
@@ -338,7 +386,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
       for (int b = 0; b < 2; ++b) hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, q32[ax][b]);
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl_in, 0.0, 0.0);
     LAUNCH_CHECK("inner reset");
-    if (lin && (rc = lin_coefs(dt, Vpdtau, a))) return rc;
+    // (the coefficient-form sweep's coefficients come with the residual pass)
     const int cap = fixed_inner > 0 ? fixed_inner : (int)std::min<int64_t>(INT_MAX - 1, std::max<int64_t>(1, total_cap - inner_total));
     int it = 0, seg = 0;
     bool pending = false;
@@ -364,7 +412,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
         ebuf ^= 1;
         ends.push_back({tl, ebuf});
         if (chk_last) {
-          hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(256), 0, stream, ctrl_in, (const Ctrl*)ctrl,
+          hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(1024), 0, stream, ctrl_in, (const Ctrl*)ctrl,
                              (const double*)(lin ? lin_psum : part_sum), lin ? lin_nblocks : nparts, N_full,
                              fixed_inner > 0 ? 0.0 : eta, tl, cap);
           LAUNCH_CHECK("k_inner_finalize");

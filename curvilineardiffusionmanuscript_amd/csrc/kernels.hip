@@ -213,23 +213,39 @@ __global__ void k_ctrl_validated(Ctrl* ctrl) {
 // partials in a fixed order, update rel/abs, and decide the exit in the
 // reference's order: non-finite, then iter > max_iter, then tolerance.
 // `check` = 0 is the max_iter-only test of a non-check iteration.
-__global__ __launch_bounds__(256) void k_finalize(Ctrl* ctrl, const double* __restrict__ psum,
-                                                  const double* __restrict__ pmax, int nparts, int iter, int max_iter,
-                                                  double rtol, double atol, double nfull, int norm_kind, int check) {
+__global__ __launch_bounds__(1024) void k_finalize(Ctrl* ctrl, const double* __restrict__ psum,
+                                                   const double* __restrict__ pmax, int nparts, int iter, int max_iter,
+                                                   double rtol, double atol, double nfull, int norm_kind, int check) {
   if (ctrl->done) return;
-  __shared__ double ss[256], sm[256];
+  constexpr int NT = 1024, U = 8;
+  __shared__ double ss[NT], sm[NT];
   const int t = threadIdx.x;
   double s = 0.0, m = 0.0;
   if (check) {
-    for (int p = t; p < nparts; p += 256) {
-      s += psum[p];
-      m = nanmax(m, pmax[p]);
+    // U independent loads in flight per thread; fixed summation order
+    double as[U], am[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) as[u] = am[u] = 0.0;
+    for (int base = 0; base < nparts; base += NT * U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int p = base + u * NT + t;
+        if (p < nparts) {
+          as[u] += psum[p];
+          am[u] = nanmax(am[u], pmax[p]);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      s += as[u];
+      m = nanmax(m, am[u]);
     }
   }
   ss[t] = s;
   sm[t] = m;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = NT / 2; o > 0; o >>= 1) {
     if (t < o) {
       ss[t] += ss[t + o];
       sm[t] = nanmax(sm[t], sm[t + o]);

@@ -243,7 +243,7 @@ int cdiff_solver::run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int 
       launches.push_back({(int)tl, 1 - buf, pair});
       buf ^= 1;
       if (chk_last || tl == limit) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, stream, ctrl, (const double*)part_sum,
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl, (const double*)part_sum,
                            (const double*)part_max, chk_last ? nparts : 0, (int)tl, o.max_iter, o.rtol, o.atol,
                            N_full, o.norm_kind, chk_last ? 1 : 0);
         LAUNCH_CHECK("k_finalize");

@@ -75,6 +75,7 @@ struct cdiff_solver {
   double* lin_psum = nullptr;
   double* lin_pmax = nullptr;
   int lin_nparts = 0;
+  double dc_Vpdtau = 0.0;
 
   // sweep decomposition
   bool tile2d = true;  // 2D: LDS-tiled kernel (else the row-marching kernel)
