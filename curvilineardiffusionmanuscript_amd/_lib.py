@@ -46,6 +46,19 @@ EXPORTS = (
     "cdiff_last_error",
     "cdiff_status_string",
     "cdiff_destroy",
+    "cdiff_net_num_params",
+    "cdiff_net_create",
+    "cdiff_net_destroy",
+    "cdiff_net_set_params",
+    "cdiff_net_get_params",
+    "cdiff_net_dataset",
+    "cdiff_net_capture",
+    "cdiff_net_train",
+    "cdiff_net_eval",
+    "cdiff_net_loss_grad",
+    "cdiff_net_last_error",
+    "cdiff_set_warm_start",
+    "cdiff_net_set_gate",
 )
 
 
@@ -84,7 +97,7 @@ class Stats(C.Structure):
         ("nchecks", C.c_int32),
         ("time", C.c_double),
         ("outer", C.c_int32),
-        ("pad", C.c_int32),
+        ("warm_start", C.c_int32),
     ]
 
 
@@ -133,8 +146,23 @@ def load(path: str = LIB_PATH):
     L.cdiff_status_string.argtypes = [C.c_int]
     L.cdiff_status_string.restype = C.c_char_p
     L.cdiff_destroy.argtypes = [_vp]
+    _fp = C.POINTER(C.c_float)
+    L.cdiff_net_num_params.argtypes = []
+    L.cdiff_net_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+    L.cdiff_net_destroy.argtypes = [_vp]
+    L.cdiff_net_set_params.argtypes = [_vp, _fp, C.c_int]
+    L.cdiff_net_get_params.argtypes = [_vp, _fp, C.c_int]
+    L.cdiff_net_dataset.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.cdiff_net_capture.argtypes = [_vp, _vp, C.c_int, C.c_int, C.c_double]
+    L.cdiff_net_train.argtypes = [_vp, C.c_int, C.c_int, C.c_float, C.c_uint64, _dp]
+    L.cdiff_net_eval.argtypes = [_vp, C.c_int, _fp, _dp]
+    L.cdiff_net_loss_grad.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_uint16), _fp, _fp, _fp, _dp]
+    L.cdiff_net_last_error.argtypes = [_vp]
+    L.cdiff_net_last_error.restype = C.c_char_p
+    L.cdiff_set_warm_start.argtypes = [_vp, _vp]
+    L.cdiff_net_set_gate.argtypes = [_vp, C.c_float]
     for name in EXPORTS:
-        if name not in ("cdiff_default_opts", "cdiff_last_error", "cdiff_status_string"):
+        if name not in ("cdiff_default_opts", "cdiff_last_error", "cdiff_status_string", "cdiff_net_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

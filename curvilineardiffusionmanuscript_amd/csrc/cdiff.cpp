@@ -293,6 +293,18 @@ int cdiff_set_metrics(cdiff_solver* h, const double* const* cell, const double* 
     else
       m.edge[t - nm] = d;
   }
+  if (nd == 2) {
+    // cell Jacobian 1 / det(d(xi, eta)/d(x, y)), a feature of the learned warm start
+    if (!h->jac) {
+      int rc = alloc_field(h, &h->jac);
+      if (rc) {
+        cleanup();
+        return rc;
+      }
+    }
+    hipLaunchKernelGGL(k_cell_jacobian2d, grid_full(h->g), dim3(256), 0, h->stream, h->g, (const double*)m.cell[0],
+                       (const double*)m.cell[1], (const double*)m.cell[2], (const double*)m.cell[3], h->jac);
+  }
   double* c[9] = {nullptr};
   for (int t = 0; t < 9; ++t) c[t] = t < (nd == 2 ? 5 : 9) ? h->coef[t] : h->coef[0];
   hipLaunchKernelGGL(k_metric_coefs, grid_full(h->g), dim3(256), 0, h->stream, h->g, m, c[0], c[1], c[2], c[3], c[4],
@@ -389,6 +401,7 @@ int cdiff_set_forcing(cdiff_solver* h, const double* A, const double* B) {
 
 int cdiff_set_temperature(cdiff_solver* h, const double* T) {
   if (!h || !T) return CDIFF_BAD_ARG;
+  h->have_prev = false;
   int rc = upload(h, h->Tdev, T);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -457,6 +470,7 @@ int cdiff_reset_state(cdiff_solver* h) {
   h->last_iter = 0;
   h->cur = 0;
   h->have_last = false;
+  h->have_prev = false;
   h->q32_valid = false;
   HIP_TRY(hipStreamSynchronize(h->stream));
   return CDIFF_OK;

@@ -14,7 +14,7 @@ __global__ void k_dc_residual(Geo g, int harmonic, double dt, const double* u, c
                               DcCoefs cf, const double* S, const double* tzA, const double* tzB, float* r32,
                               double* psum, double* pmax);
 __global__ void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* psum, int nparts, double nfull, double eta,
-                                 int iter, int cap);
+                                 int iter, int cap, double floor_frac, double rtol, double atol);
 __global__ void k_fill32(int64_t n, float v, float* dst);
 __global__ void k_lin_metrics(Geo g, const double* g11, const double* g22, const double* c12, const double* ac,
                               const double* bc, float* m0, float* m1, float* m2, float* m3, float* m4);

@@ -192,7 +192,8 @@ __global__ __launch_bounds__(256) void k_dc_residual(Geo g, int harmonic, double
 // Inner-loop decision: stop the FP32 sweeps once the correction residual has
 // dropped below eta * (the outer FP64 residual), or after `cap` sweeps.
 __global__ __launch_bounds__(1024) void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* __restrict__ psum,
-                                                         int nparts, double nfull, double eta, int iter, int cap) {
+                                                         int nparts, double nfull, double eta, int iter, int cap,
+                                                         double floor_frac, double rtol, double atol) {
   if (cin->done) return;
   constexpr int NT = 1024, U = 8;
   __shared__ double ss[NT];
@@ -219,7 +220,11 @@ __global__ __launch_bounds__(1024) void k_inner_finalize(Ctrl* cin, const Ctrl* 
   const double L2 = sqrt(ss[0] / nfull);
   cin->last_l2 = L2;
   cin->nchecks += 1;
-  if (!(L2 <= 1e300) || L2 <= eta * cout->last_l2 || iter >= cap) {
+  // stop at eta times the outer residual, or once the outer tolerance is in
+  // reach: a fraction of max(atol, rtol * initial residual)
+  double target = eta * cout->last_l2;
+  if (floor_frac > 0.0) target = fmax(target, floor_frac * fmax(atol, rtol * cout->init));
+  if (!(L2 <= 1e300) || L2 <= target || iter >= cap) {
     cin->done = 1;
     cin->final_iter = iter;
     cin->status = (L2 == L2) ? 0 : 1;

@@ -22,6 +22,8 @@ __global__ void k_iter_params_full(Geo g, IterConst ic, const double* rho, const
 template <typename T>
 __global__ void k_convert(Geo g, const double* src, T* dst);
 __global__ void k_copy(int64_t n, const double* src, double* dst);
+__global__ void k_cell_jacobian2d(Geo g, const double* xx, const double* xy, const double* ex, const double* ey,
+                                  double* jac);
 __global__ void k_merge(Geo g, const double* dom_src, const double* rest, double* out);
 __global__ void k_fill(int64_t n, double v, double* dst);
 __global__ void k_validate(Geo g, const double* a, int positivity, int shift, Ctrl* ctrl);

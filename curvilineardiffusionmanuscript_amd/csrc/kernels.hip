@@ -477,4 +477,17 @@ __global__ void k_residual_field(Geo g, double dt, const double* __restrict__ u,
   res[c] = r;
 }
 
+// Cell Jacobian of a 2D mesh from the cell metrics: 1 / (xi_x eta_y - xi_y eta_x).
+__global__ void k_cell_jacobian2d(Geo g, const double* __restrict__ xx, const double* __restrict__ xy,
+                                  const double* __restrict__ ex, const double* __restrict__ ey,
+                                  double* __restrict__ jac) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t ntot = (int64_t)g.n[0] * g.n[1];
+  if (e >= ntot) return;
+  const int i = (int)(e % g.n[0]), j = (int)(e / g.n[0]);
+  const int64_t c = at(g, i, j, 0);
+  const double det = xx[c] * ey[c] - xy[c] * ex[c];
+  jac[c] = det != 0.0 ? 1.0 / det : 0.0;
+}
+
 }  // namespace cdiff

@@ -13,6 +13,7 @@
 
 #include "kernels.h"
 #include "solver.h"
+#include "net.h"
 #include "sweep_args.h"
 
 using namespace cdiff;
@@ -346,6 +347,21 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   last_subcycle = o.subcycle_conductivity;
   cdiff_stats st{};
 
+  // learned warm start: features of (T^n, T^{n-1}) and the prediction, before
+  // the prologue overwrites u_prev
+  warm_pending = false;
+  if (net && o.precision == CDIFF_PREC_FP32_DC && lin_ok()) {
+    if (!learn_ts) {
+      HIP_TRY(hipMalloc((void**)&learn_ts, 2 * sizeof(double)));
+      owned.push_back(learn_ts);
+    }
+    int rc = net->workspace(Img{g.hi[1] - g.lo[1] + 1, g.hi[0] - g.lo[0] + 1});
+    if (rc) return fail(rc, net->err);
+    if ((rc = learn_features(dt, net->X0, learn_ts))) return rc;
+    if ((rc = net->forward(net->X0, stream))) return fail(rc, net->err);
+    warm_pending = true;
+  }
+
   // prologue: u = u_prev = T (PseudoTransientSolver.jl:229-230), validity (:238-250)
   for (double* d : {u[0], u[1], u_prev}) {
     hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)Tdev, d);
@@ -464,6 +480,8 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   last_src = src;
   last_pre = dc ? u[dst] : pre;
   have_last = true;
+  have_prev = true;  // u_prev = T^n, the previous level of the next step
+  dt_prev = dt;
   t_written = true;
   const auto t1 = std::chrono::steady_clock::now();
   st.time = std::chrono::duration<double>(t1 - t0).count();

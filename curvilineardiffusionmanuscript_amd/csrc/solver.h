@@ -102,9 +102,25 @@ struct cdiff_solver {
   };
   std::vector<Launch> launches;  // of the current step
 
+  // learned warm start (2D FP32_DC steps)
+  cdiff_net* net = nullptr;
+  double* jac = nullptr;         // cell Jacobian (2D), for the network's features
+  bool have_prev = false;        // u_prev holds the previous time level T^{n-1}
+  double dt_prev = 0.0;
+  double* learn_scratch = nullptr;  // >= 1024 doubles: reductions of the feature pass
+  double* learn_ts = nullptr;       // device: [max |T^n|, dt max |r(T^n)|] of the current step
+  bool warm_pending = false;        // a prediction for this step is in the network's workspace
+
   std::string err;
 
   int fail(int code, const std::string& msg);
+  bool lin_ok() const;
+  // features of a step of size dt from the resident temperature (Tdev = T^n,
+  // u_prev = T^{n-1} when have_prev) into X0; the feature scales into ts_dev[0..1]
+  int learn_features(double dt, void* X0, double* ts_dev);
+  // (Tdev - u_prev) / ts_dev[1] after a step
+  int learn_target(const double* ts_dev, float* tgt);
+  int warm_start_apply(double dt, const cdiff_opts& o, bool* used);
   int hip_fail(hipError_t e, const char* what);
   void configure_sweep();
   int step(double dt, const cdiff_opts& o, cdiff_stats* stats, double* next_dt);
@@ -121,4 +137,7 @@ struct cdiff_solver {
   int bench_sweeps(double dt, int precision, int check, int n, double* mean_ms);
   int ensure_fp32();
   int dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o);
+  // the fused outer pass on explicit buffers (2D, lin_ok); finalize: update ctrl
+  int dc_outer_pass(double dt, const double* u_old, const float* e, double* u_new, const double* uprev, int iter,
+                    const cdiff_opts& o, bool finalize);
 };

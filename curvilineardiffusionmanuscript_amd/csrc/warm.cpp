@@ -1,0 +1,103 @@
+// warm.cpp -- the solver side of the learned warm start: features of the
+// resident state, training targets, and the start of an FP32_DC step from the
+// network's prediction (with the fallback to T^n).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+
+#include "dc.h"
+#include "kernels.h"
+#include "learn.h"
+#include "net.h"
+#include "solver.h"
+
+using namespace cdiff;
+
+#define HIP_TRY(expr)                                 \
+  do {                                                \
+    hipError_t _e = (expr);                           \
+    if (_e != hipSuccess) return hip_fail(_e, #expr); \
+  } while (0)
+
+#define LAUNCH_CHECK(name)                                        \
+  do {                                                            \
+    hipError_t _e = hipGetLastError();                            \
+    if (_e != hipSuccess) return hip_fail(_e, "launch of " name); \
+  } while (0)
+
+int cdiff_solver::learn_features(double dt, void* X0, double* ts_dev) {
+  if (!metrics_set) return fail(CDIFF_NOT_READY, "cdiff_set_metrics has not been called");
+  if (!lin_ok() || !jac) return fail(CDIFF_BAD_ARG, "the learned warm start needs a 2D mesh without periodic faces");
+  if (!(dt > 0)) return fail(CDIFF_BAD_ARG, "dt > 0 required");
+  int rc = ensure_fp32();
+  if (rc) return rc;
+  if (!learn_scratch) {
+    HIP_TRY(hipMalloc((void**)&learn_scratch, sizeof(double) * 1024));
+    owned.push_back(learn_scratch);
+  }
+  cdiff_opts o;
+  cdiff_default_opts(&o);
+  double smin = spacing[0];
+  for (int a = 1; a < ndim; ++a) smin = std::min(smin, spacing[a]);
+  dc_Vpdtau = o.cfl * smin;
+  // alpha(T^n) and the FP64 residual of T^n as the start of a step (u = u_prev = T^n)
+  if ((rc = dc_outer_pass(dt, Tdev, nullptr, u[1], Tdev, 1, o, false))) return rc;
+  launch_domain_maxabs(g, Tdev, learn_scratch, ts_dev, stream);
+  launch_domain_maxabs32(g, base32, dt, learn_scratch, ts_dev + 1, stream);
+  LAUNCH_CHECK("feature scales");
+  FeatIn f{};
+  f.g = g;
+  f.harmonic = mean == CDIFF_MEAN_HARMONIC;
+  f.dt = dt;
+  f.dt_ratio = (have_prev && dt_prev > 0) ? dt / dt_prev : 0.0;
+  f.j_ref = spacing[0] * spacing[1];
+  f.u = Tdev;
+  f.um1 = have_prev ? u_prev : Tdev;
+  f.a = a64;
+  f.g11 = coef[0];
+  f.g22 = coef[1];
+  f.jac = jac;
+  f.r32 = base32;
+  f.scales = ts_dev;
+  launch_features(f, (__bf16*)X0, stream);
+  LAUNCH_CHECK("k_features");
+  return CDIFF_OK;
+}
+
+int cdiff_solver::learn_target(const double* ts_dev, float* tgt) {
+  if (!have_prev) return fail(CDIFF_NOT_READY, "no completed step to take a target from");
+  launch_target(g, Tdev, u_prev, ts_dev, tgt, stream);
+  LAUNCH_CHECK("k_target");
+  return CDIFF_OK;
+}
+
+// After the first outer pass of an FP32_DC step (u[1] = T^n with its BC
+// ghosts, its residual norm in ctrl_h[2]): evaluate the residual of the
+// predicted start and keep whichever start has the smaller residual.
+int cdiff_solver::warm_start_apply(double dt, const cdiff_opts& o, bool* used) {
+  *used = false;
+  const double r0 = ctrl_h[2].abs_;
+  launch_apply_prediction(g, u[1], net->delta, learn_ts, u[0], stream);
+  LAUNCH_CHECK("k_apply_prediction");
+  int rc = dc_residual(dt, 0, nullptr, 1, 1, o);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (std::getenv("CDIFF_DC_DEBUG"))
+    std::fprintf(stderr, "[dc] residual of the predicted start %.3e (of T^n %.3e)\n", ctrl_h[2].abs_, r0);
+  if (ctrl_h[2].abs_ < r0 && ctrl_h[2].status == CDIFF_OK) {
+    *used = true;
+    return CDIFF_OK;
+  }
+  // fallback: start from T^n (in u_prev) again
+  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, o.rtol, o.atol);
+  hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)u_prev, u[0]);
+  LAUNCH_CHECK("warm start fallback");
+  if ((rc = dc_residual(dt, 0, nullptr, 1, 1, o))) return rc;
+  HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  return CDIFF_OK;
+}

@@ -123,7 +123,7 @@ typedef struct {
   int32_t nchecks; /* residual evaluations */
   double time;     /* seconds, device-synchronised wall time of the step */
   int32_t outer;   /* FP32_DC: FP64 defect-correction rounds */
-  int32_t pad;
+  int32_t warm_start; /* 1 if the step started from the learned prediction */
 } cdiff_stats;
 
 /* Fields for cdiff_get_field / cdiff_set_field. */
@@ -224,6 +224,53 @@ int cdiff_reset_state(cdiff_solver *h);
 int cdiff_bench_sweeps(cdiff_solver *h, double dt, int precision, int check, int nsweeps, double *mean_ms);
 
 const char *cdiff_last_error(const cdiff_solver *h);
+
+/* ---------------------------------------------------------------------------
+ * Learned warm start (no counterpart in the reference).  A 4-layer 3x3
+ * convolutional network (8 -> 32 -> 32 -> 32 -> 1 channels, ReLU, BF16
+ * weights and activations, FP32 accumulation on MFMA) predicts
+ * (T^{n+1} - T^n) / max|T^n| per cell from features of the solver state at
+ * the start of a step: T^n, the previous increment, the cell Jacobian, the
+ * face diffusion numbers dt*alpha_face*g, and the explicit-Euler increment
+ * dt*r(T^n).  With a network attached, an FP32_DC step of a 2D mesh without
+ * periodic faces starts from T^n + prediction when that start has the smaller
+ * FP64 residual, and from T^n otherwise; it always iterates to the same
+ * tolerance.  The network is trained on the GPU by its own kernels on step
+ * pairs captured from the solver. */
+typedef struct cdiff_net cdiff_net;
+
+int cdiff_net_num_params(void);
+/* New network, weights initialised from `seed` (He initialisation). */
+int cdiff_net_create(uint64_t seed, cdiff_net **out);
+int cdiff_net_destroy(cdiff_net *net);
+int cdiff_net_set_params(cdiff_net *net, const float *params, int n);
+int cdiff_net_get_params(cdiff_net *net, float *params, int n);
+/* Training store in device memory: `capacity` samples of height x width
+ * cells (features + target). */
+int cdiff_net_dataset(cdiff_net *net, int height, int width, int capacity);
+/* Capture from a solver whose resident temperature is T^n: stage 0 stores the
+ * features of a step of size dt in `slot`; stage 1, after that step, the
+ * target (T^{n+1} - T^n) / max|T^n|. */
+int cdiff_net_capture(cdiff_net *net, cdiff_solver *h, int slot, int stage, double dt);
+/* `epochs` passes of Adam (lr, beta 0.9/0.999) over samples [0, nsamples) in
+ * a seeded order; mean squared error of the last epoch in *loss. */
+int cdiff_net_train(cdiff_net *net, int nsamples, int epochs, float lr, uint64_t seed, double *loss);
+/* Prediction and mean squared error of a stored sample (delta: height*width
+ * floats, row-major by j, or NULL). */
+int cdiff_net_eval(cdiff_net *net, int slot, float *delta, double *loss);
+/* Loss and its gradient for host-provided features (bf16 bit patterns,
+ * (height+2)*(width+2)*8 with a zero border) and target (height*width):
+ * grad has cdiff_net_num_params() floats, delta height*width (either may be
+ * NULL).  Test hook for the backward kernels. */
+int cdiff_net_loss_grad(cdiff_net *net, int height, int width, const uint16_t *features, const float *target,
+                        float *grad, float *delta, double *loss);
+const char *cdiff_net_last_error(const cdiff_net *net);
+/* Activity threshold: the prediction is 0 (and carries no gradient) on cells
+ * whose 3x3 neighbourhood has |dt r(T^n)| <= gate * dt max|r(T^n)|
+ * (default 1e-5).  Part of the trained model: set it before training. */
+int cdiff_net_set_gate(cdiff_net *net, float gate);
+/* Attach (or detach, net = NULL) the warm start to a solver. */
+int cdiff_set_warm_start(cdiff_solver *h, cdiff_net *net);
 const char *cdiff_status_string(int status);
 int cdiff_destroy(cdiff_solver *h);
 

@@ -1,0 +1,95 @@
+"""The learned warm start on the GPU: the network's MFMA forward/backward
+kernels against the CPU oracle (oracle/net_oracle.py), training on captured
+APT step pairs, and FP32 defect-correction steps started from the prediction.
+
+Tolerances: the kernels store activations and pre-activation gradients in
+bfloat16 (8 significant bits) and accumulate in FP32; the oracle rounds the
+same quantities to bfloat16 and accumulates in float64.  A bfloat16 rounding
+that lands on the other side of a tie, or an FP32 sum that differs in its
+last bit, changes a stored value by one bf16 ulp (2^-8 relative), so the
+comparison is norm-wise: 2e-2 relative per parameter block, 1e-2 on the
+prediction."""
+import numpy as np
+import pytest
+
+import net_oracle as N
+from cases import hot_disk_2d
+from curvilineardiffusionmanuscript_amd import learned as LW
+from curvilineardiffusionmanuscript_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _blocks(g):
+    offs = np.cumsum([0] + [n for _, n in N.SIZES])
+    return {name: g[offs[k] : offs[k + 1]] for k, (name, _) in enumerate(N.SIZES)}
+
+
+@pytest.mark.parametrize("H,W", [(37, 70), (8, 16), (64, 64)])
+def test_loss_gradient_matches_oracle(H, W):
+    p = N.init_params(11)
+    X = N.random_features(H, W, 12)
+    X[1 : H // 2 + 1, 1 : W // 3 + 1, 5] = 0.0  # an inactive region: prediction and gradient 0 there
+    Xb = N.bf16_round(X)
+    t = np.random.default_rng(13).normal(0, 0.5, (H, W)).astype(np.float32)
+    net = LW.WarmStartNet()
+    net.params = p
+    loss, g, d = net.loss_grad(N.bf16_bits(X), t)
+    lo, go, do = N.loss_grad(p.astype(np.float64), Xb.astype(np.float64), t.astype(np.float64), bf16=True)
+    print(f"{H}x{W}: loss gpu {loss:.6e} oracle {lo:.6e}")
+    assert abs(loss - lo) <= 1e-2 * lo
+    assert np.linalg.norm(d - do) <= 1e-2 * np.linalg.norm(do)
+    gb, ob = _blocks(g), _blocks(go)
+    for name, _ in N.SIZES:
+        err = np.linalg.norm(gb[name] - ob[name]) / max(np.linalg.norm(ob[name]), 1e-30)
+        print(f"  {name}: rel err {err:.2e}")
+        assert err <= 2e-2, name
+    net.close()
+
+
+def test_training_on_captured_pairs_reduces_loss():
+    n, full = 48, 512
+    net = LW.WarmStartNet(seed=1)
+    net.dataset(n, n, 12)
+    got = LW.generate_pairs(net, n=n, full=full, nseeds=3, steps=5, dt0=2e-5)
+    assert got == 12
+    _, l0 = net.evaluate(0)
+    loss = net.train(epochs=40, lr=2e-3, seed=0)
+    _, l1 = net.evaluate(0)
+    print(f"loss before {l0:.3e}, last epoch {loss:.3e}, sample 0 after {l1:.3e}")
+    assert l1 < 0.3 * l0
+    net.close()
+
+
+def test_warm_started_step_reaches_the_same_tolerance():
+    # train briefly, then step a held-out problem with and without the warm start
+    n, full = 64, 512
+    net = LW.WarmStartNet(seed=2)
+    net.dataset(n, n, 24)
+    LW.generate_pairs(net, n=n, full=full, nseeds=4, steps=7, dt0=2e-5, seed0=100)
+    net.train(epochs=60, lr=2e-3, seed=1)
+    mesh, T0 = hot_disk_2d(n, seed=5)
+    bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
+    law = S.PowerLawConductivity(1.0, 2.5, 0.0)
+    out = {}
+    for warm in (False, True):
+        sol = S.PseudoTransientSolver(mesh, bcs)
+        if warm:
+            net.attach(sol)
+        S.update_conductivity(sol, mesh, T0, 1.0, 1.0, law)
+        sol.set_temperature(T0)
+        dt, its, used, errs = 2e-5, [], [], []
+        for _ in range(6):
+            st, dt = sol.step_resident(dt, None, 1.0, law, rtol=0.0, atol=1e-8, max_iter=10**6, precision="fp32_dc")
+            its.append(st.niter)
+            used.append(st.warm_start)
+            errs.append(st.abs_err)
+        out[warm] = (its, used, errs, sol.get_temperature())
+        sol.close()
+    print("cold", out[False][:2], "warm", out[True][:2])
+    assert not any(out[False][1])
+    assert all(e <= 1e-8 for e in out[True][2])
+    # both trajectories satisfy the same FP64 residual tolerance every step
+    T_cold, T_warm = out[False][3], out[True][3]
+    assert np.max(np.abs(T_warm - T_cold)) <= 6 * 2e-5 * np.sqrt(T0.size) * 1e-8 * 10
+    net.close()
