@@ -113,3 +113,36 @@ def test_lin_sweep_depth_does_not_change_result(depth, monkeypatch):
         out.append((its, T))
     assert out[0][0] == out[1][0]
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+def test_c3_mixed_precision_vs_implicit_direct_512():
+    """Mixed precision at 512^2 (config-1 physics: wavy grid A = 0.03, kappa = 1,
+    Gaussian T0, zero flux, dt = 1e-4): per-step error against the FP64
+    implicit-direct oracle (oracle/implicit_oracle.py, a SciPy direct solve of
+    the reference's conservative scheme).  The two are different consistent
+    discretizations, so they agree to O(h^2) in the interior (the boundary
+    band is excluded, see tests/test_implicit_oracle.py); the FP32 sweeps add
+    nothing measurable on top of the FP64 APT."""
+    import implicit_oracle as I
+
+    n, dt = 512, 1e-4
+    mesh, T0 = gaussian_2d(n)
+    law = S.PowerLawConductivity(1.0)
+    dev = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4)
+    imp = I.ImplicitOracle2D(mesh, [("neumann", 0.0)] * 4)
+    Td, Ti = T0.copy(order="F"), T0.copy(order="F")
+    S.update_conductivity(dev, mesh, Td, 1.0, 1.0, law)
+    q = slice(1 + n // 4, 1 + 3 * n // 4)
+    for k in range(2):
+        Tprev = Td.copy()
+        sd, _ = S.nonlinear_thermal_conduction_step(dev, mesh, Td, 1.0, 1.0, law, dt, rtol=0.0, atol=1e-8,
+                                                    max_iter=10**6, precision="fp32_dc", calculate_next_dt=False,
+                                                    show_convergence=False)
+        imp.step(Ti, dt, cutoff=False)
+        err = float(np.max(np.abs(Td - Ti)[q, q]))
+        inc = float(np.max(np.abs(Td - Tprev)[q, q]))
+        print(f"512^2 step {k}: dc inner={sd.niter} outer={sd.outer}; max|T_dc - T_implicit| = {err:.3e} "
+              f"(step increment {inc:.3e}, ratio {err / inc:.2e})")
+        assert err <= 1e-5 * inc
+        Ti[...] = Td  # continue both from the same state
+    dev.close()
