@@ -119,7 +119,7 @@ int cdiff_solver::ensure_fp32() {
   lin = lin_ok();
   if (lin) {
     lin_k = env_int("CDIFF_LIN_K", 4);
-    if (lin_k != 2 && lin_k != 4 && lin_k != 6) lin_k = 4;
+    if (lin_k != 2 && lin_k != 4 && lin_k != 6 && lin_k != 8) lin_k = 4;
     const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
     const int rows = lin2d_rows(lin_k);
     lin_chunks = (nj + rows - 1) / rows;
@@ -379,7 +379,9 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   const int64_t total_cap = (int64_t)o.max_iter;
   const bool pairs = pairs_ok();
   const int lin_nit = std::max(1, std::min(lin_k, env_int("CDIFF_LIN_NIT", lin_k)));
-  dc_Vpdtau = Vpdtau;
+  // pseudo-time step of the correction sweeps: the fixed point does not depend
+  // on it, so it may differ from the FP64 APT's (scale 1 = the reference's)
+  dc_Vpdtau = Vpdtau * env_double("CDIFF_DC_CFL_SCALE", 1.0);
   // u[0] holds T (prologue); the first FP64 residual, of T with its BCs, into u[1]
   int a = 1;
   if ((rc = dc_residual(dt, 0, nullptr, a, 1, o))) return rc;

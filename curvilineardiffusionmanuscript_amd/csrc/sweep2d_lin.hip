@@ -4,8 +4,9 @@
 // Inside one outer iteration the correction equation is linear: alpha, theta
 // and dtau are frozen at the FP64 iterate, and the Newton term of the face
 // conductivity is linear in the correction e.  So everything that does not
-// depend on e is computed once per outer iteration, in FP64, by k_lin_coefs
-// (dc_kernels.hip), and stored per face and per cell in FP32:
+// depend on e is computed once per outer iteration, in FP64, by k_dc_outer2d
+// (dc_outer2d.hip; k_lin_coefs in dc_kernels.hip does the same on its own),
+// and stored per face and per cell in FP32:
 //
 //   face f between cells 0 and 1:  q'_f = P_f e_0 - M_f e_1
 //                                   q_f <- q'_f + W_f (q_f - q'_f)
@@ -33,11 +34,10 @@ namespace cdiff {
 namespace {
 
 constexpr int LTX = 64;
-constexpr int LNTHR = 512;
 
 }  // namespace
 
-template <int K, int TY, bool CHECK>
+template <int K, int TY, int LNTHR, bool CHECK>
 __global__ __launch_bounds__(LNTHR) void sweep2d_lin_kernel(Lin2DArgs A) {
   constexpr int RX = LTX + 2 * K, RY = TY + 2 * K, NR = RX * RY;
   constexpr int NS = (NR + LNTHR - 1) / LNTHR;
@@ -204,21 +204,22 @@ __global__ __launch_bounds__(LNTHR) void sweep2d_lin_kernel(Lin2DArgs A) {
   }
 }
 
-template <int K, int TY>
+template <int K, int TY, int NT>
 static void launch_k(const Lin2DArgs& A, bool check, int nblocks, hipStream_t s) {
   if (check)
-    hipLaunchKernelGGL((sweep2d_lin_kernel<K, TY, true>), dim3(nblocks), dim3(LNTHR), 0, s, A);
+    hipLaunchKernelGGL((sweep2d_lin_kernel<K, TY, NT, true>), dim3(nblocks), dim3(NT), 0, s, A);
   else
-    hipLaunchKernelGGL((sweep2d_lin_kernel<K, TY, false>), dim3(nblocks), dim3(LNTHR), 0, s, A);
+    hipLaunchKernelGGL((sweep2d_lin_kernel<K, TY, NT, false>), dim3(nblocks), dim3(NT), 0, s, A);
 }
 
 int lin2d_rows(int kmax) { return kmax >= 4 ? 16 : 8; }
 
 void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, bool check, int nblocks, hipStream_t s) {
   switch (kmax) {
-    case 4: launch_k<4, 16>(A, check, nblocks, s); break;
-    case 6: launch_k<6, 16>(A, check, nblocks, s); break;
-    default: launch_k<2, 8>(A, check, nblocks, s); break;
+    case 4: launch_k<4, 16, 512>(A, check, nblocks, s); break;
+    case 6: launch_k<6, 16, 1024>(A, check, nblocks, s); break;
+    case 8: launch_k<8, 16, 1024>(A, check, nblocks, s); break;
+    default: launch_k<2, 8, 512>(A, check, nblocks, s); break;
   }
 }
 

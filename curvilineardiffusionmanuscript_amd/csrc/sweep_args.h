@@ -122,7 +122,7 @@ struct Lin2DArgs {
   const Ctrl* ctrl;
 };
 
-// kmax: 2, 4 or 6 iterations per launch at most
+// kmax: 2, 4, 6 or 8 iterations per launch at most
 int lin2d_rows(int kmax);
 void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, bool check, int nblocks, hipStream_t s);
 

@@ -88,7 +88,7 @@ def test_dc_then_fp64_step_continues():
     assert st.abs_err <= 1e-8 and st.niter > 0
 
 
-@pytest.mark.parametrize("depth", ["2", "4", "6"])
+@pytest.mark.parametrize("depth", ["2", "4", "6", "8"])
 def test_lin_sweep_depth_does_not_change_result(depth, monkeypatch):
     # the coefficient-form sweep runs up to `depth` iterations per launch on a
     # tile with a depth-cell ring; per-cell arithmetic does not depend on how
