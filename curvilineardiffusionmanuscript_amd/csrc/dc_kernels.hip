@@ -224,10 +224,13 @@ __global__ __launch_bounds__(1024) void k_inner_finalize(Ctrl* cin, const Ctrl* 
   // reach: a fraction of max(atol, rtol * initial residual)
   double target = eta * cout->last_l2;
   if (floor_frac > 0.0) target = fmax(target, floor_frac * fmax(atol, rtol * cout->init));
-  if (!(L2 <= 1e300) || L2 <= target || iter >= cap) {
+  // growth far above the starting residual (that of e = 0, the outer residual):
+  // the sweeps are unstable for these pseudo-time parameters
+  const bool diverged = L2 > 1e2 * cout->last_l2;
+  if (!(L2 <= 1e300) || diverged || L2 <= target || iter >= cap) {
     cin->done = 1;
     cin->final_iter = iter;
-    cin->status = (L2 == L2) ? 0 : 1;
+    cin->status = !(L2 == L2) ? 1 : (diverged || !(L2 <= 1e300) ? 3 : 0);
   }
 }
 

@@ -381,7 +381,8 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   const int lin_nit = std::max(1, std::min(lin_k, env_int("CDIFF_LIN_NIT", lin_k)));
   // pseudo-time step of the correction sweeps: the fixed point does not depend
   // on it, so it may differ from the FP64 APT's (scale 1 = the reference's)
-  dc_Vpdtau = Vpdtau * env_double("CDIFF_DC_CFL_SCALE", 1.0);
+  if (dc_cfl_scale < 0) dc_cfl_scale = env_double("CDIFF_DC_CFL_SCALE", ndim == 2 ? 1.2 : 1.0);
+  dc_Vpdtau = Vpdtau * dc_cfl_scale;
   // u[0] holds T (prologue); the first FP64 residual, of T with its BCs, into u[1]
   int a = 1;
   if ((rc = dc_residual(dt, 0, nullptr, a, 1, o))) return rc;
@@ -462,6 +463,19 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     const int ni = ctrl_h[3].final_iter > 0 ? ctrl_h[3].final_iter : it;
     inner_total += ni;
     inner_checks += ctrl_h[3].nchecks;
+    if (ctrl_h[3].status != 0 && dc_cfl_scale > 1.0) {
+      // the sweeps diverged with the enlarged pseudo-time step: drop the
+      // correction, fall back to the reference's step for this solver and
+      // rebuild the coefficients of the current iterate
+      if (dc_debug) std::fprintf(stderr, "[dc] correction sweeps diverged; pseudo-time scale 1 from here\n");
+      dc_cfl_scale = 1.0;
+      dc_Vpdtau = Vpdtau;
+      if ((rc = dc_residual(dt, a, nullptr, 1 - a, outer + 2, o))) return rc;
+      a = 1 - a;
+      HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+      continue;
+    }
     int eb = ends.empty() ? 0 : ends.back().second;
     for (const auto& pr : ends)
       if (pr.first == ni) eb = pr.second;

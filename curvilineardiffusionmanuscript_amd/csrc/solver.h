@@ -76,6 +76,7 @@ struct cdiff_solver {
   double* lin_pmax = nullptr;
   int lin_nparts = 0;
   double dc_Vpdtau = 0.0;
+  double dc_cfl_scale = -1.0;  // pseudo-time step scale of the correction sweeps (< 0: not chosen yet)
 
   // sweep decomposition
   bool tile2d = true;  // 2D: LDS-tiled kernel (else the row-marching kernel)

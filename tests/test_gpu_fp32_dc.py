@@ -146,3 +146,26 @@ def test_c3_mixed_precision_vs_implicit_direct_512():
         assert err <= 1e-5 * inc
         Ti[...] = Td  # continue both from the same state
     dev.close()
+
+
+def test_unstable_pseudo_time_step_falls_back(monkeypatch):
+    # an enlarged pseudo-time step that makes the correction sweeps diverge is
+    # detected at the next inner check, dropped, and the step completes with the
+    # reference's step to the same FP64 tolerance
+    mesh, T0 = hot_disk_2d(128, seed=2)
+    bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
+    law = S.PowerLawConductivity(1.0, 2.5, 0.0)
+    res = {}
+    for scale in ("1.0", "3.0"):
+        monkeypatch.setenv("CDIFF_DC_CFL_SCALE", scale)
+        dev = S.PseudoTransientSolver(mesh, bcs)
+        T = T0.copy(order="F")
+        S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
+        for _ in range(2):
+            st, _ = S.nonlinear_thermal_conduction_step(dev, mesh, T, 1.0, 1.0, law, 1e-5, rtol=0.0, atol=1e-8,
+                                                        max_iter=10**6, precision="fp32_dc",
+                                                        calculate_next_dt=False, show_convergence=False)
+            assert st.abs_err <= 1e-8
+        res[scale] = T
+        dev.close()
+    assert np.max(np.abs(res["3.0"] - res["1.0"])) <= 2 * 1e-5 * np.sqrt(T0.size) * 1e-8 * 4
