@@ -21,8 +21,11 @@ __global__ void k_lin_metrics(Geo g, const double* g11, const double* g22, const
 struct LinOut {
   float* fx[3];
   float* fy[3];
+  float* fz[3];
   float* cc;
 };
+__global__ void k_lin_metrics3d(Geo g, DcCoefs cf, float* m0, float* m1, float* m2, float* m3, float* m4, float* m5,
+                                float* m6, float* m7, float* m8);
 // Fused outer pass of the 2D step (dc_outer2d.hip): u_new = u_old + e with
 // local BC rules (no periodic faces), alpha, the FP64 residual (into r32 and
 // block partials over 64 x 8 tiles), and the correction-sweep coefficients.

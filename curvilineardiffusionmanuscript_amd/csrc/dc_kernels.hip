@@ -297,27 +297,34 @@ __global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double 
                             const double* __restrict__ rho, const double* __restrict__ u,
                             const double* __restrict__ a, LinOut out) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t ntot = (int64_t)g.n[0] * g.n[1];
+  const int64_t ntot = (int64_t)g.n[0] * g.n[1] * g.n[2];
   if (t >= ntot) return;
-  const int i = (int)(t % g.n[0]), j = (int)(t / g.n[0]);
-  const int64_t c = at(g, i, j, 0);
-  const bool dom = i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1];
+  int idx[3];
+  unflat3(g, t, idx[0], idx[1], idx[2]);
+  const int64_t c = at(g, idx[0], idx[1], idx[2]);
+  bool dom = true, near = true;
+  for (int d = 0; d < g.ndim; ++d) {
+    dom = dom && idx[d] >= g.lo[d] && idx[d] <= g.hi[d];
+    near = near && idx[d] >= g.lo[d] - 1 && idx[d] <= g.hi[d] + 1;
+  }
   auto params = [&](int64_t cell, double& dtau, double& theta) {
     iter_params<double>(ic, rho ? rho[cell] : 1.0, a[cell], dtau, theta);
   };
   double dtau0 = 0.0, th0 = 0.0;
-  const bool need0 = i >= g.lo[0] - 1 && i <= g.hi[0] + 1 && j >= g.lo[1] - 1 && j <= g.hi[1] + 1;
-  if (need0) params(c, dtau0, th0);
+  if (near) params(c, dtau0, th0);
   out.cc[c] = dom ? (float)(dtau0 / (1.0 + dtau0 / ic.dt)) : 0.f;
-  for (int ax = 0; ax < 2; ++ax) {
-    float* const* F = ax == 0 ? out.fx : out.fy;
-    const bool on = ax == 0 ? (i >= g.lo[0] - 1 && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1])
-                            : (i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] - 1 && j <= g.hi[1]);
+  for (int ax = 0; ax < g.ndim; ++ax) {
+    float* const* F = ax == 0 ? out.fx : (ax == 1 ? out.fy : out.fz);
+    bool on = true;
+    for (int d = 0; d < g.ndim; ++d) {
+      const int lo = g.lo[d] - (d == ax ? 1 : 0);
+      on = on && idx[d] >= lo && idx[d] <= g.hi[d];
+    }
     if (!on) {
       F[0][c] = F[1][c] = F[2][c] = 0.f;
       continue;
     }
-    const int64_t c1 = ax == 0 ? at(g, i + 1, j, 0) : at(g, i, j + 1, 0);
+    const int64_t c1 = c + (ax == 0 ? (int64_t)1 : (ax == 1 ? g.pitch : g.plane));
     double dtau1, th1;
     params(c1, dtau1, th1);
     const double a0 = a[c], a1 = a[c1];
@@ -336,12 +343,10 @@ __global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double 
       M += d1 * dalpha_du(t_exp, a1, u[c1]) * du;
     }
     // boundary faces: cell 0 below the domain, or cell 1 above it
-    const int lo = ax == 0 ? g.lo[0] : g.lo[1], hi = ax == 0 ? g.hi[0] : g.hi[1];
-    const int idx = ax == 0 ? i : j;
-    if (idx == lo - 1) {
+    if (idx[ax] == g.lo[ax] - 1) {
       M -= ghost_factor(gh, 2 * ax) * P;
       P = 0.0;
-    } else if (idx == hi) {
+    } else if (idx[ax] == g.hi[ax]) {
       P -= ghost_factor(gh, 2 * ax + 1) * M;
       M = 0.0;
     }
@@ -349,6 +354,34 @@ __global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double 
     F[1][c] = (float)M;
     F[2][c] = (float)(tbar / (1.0 + tbar));
   }
+}
+
+// Divergence coefficients of the 3D correction sweep, folded per cell.
+__global__ void k_lin_metrics3d(Geo g, DcCoefs cf, float* __restrict__ m0, float* __restrict__ m1,
+                                float* __restrict__ m2, float* __restrict__ m3, float* __restrict__ m4,
+                                float* __restrict__ m5, float* __restrict__ m6, float* __restrict__ m7,
+                                float* __restrict__ m8) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t ntot = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  if (t >= ntot) return;
+  int i, j, k;
+  unflat3(g, t, i, j, k);
+  const int64_t c = at(g, i, j, k);
+  const bool dom = i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1] && k >= g.lo[2] && k <= g.hi[2];
+  if (!dom) {
+    m0[c] = m1[c] = m2[c] = m3[c] = m4[c] = m5[c] = m6[c] = m7[c] = m8[c] = 0.f;
+    return;
+  }
+  const double ha = 0.5 * cf.c[6][c], hb = 0.5 * cf.c[7][c], hg = 0.5 * cf.c[8][c];
+  m0[c] = (float)(cf.c[0][c] + ha);
+  m1[c] = (float)(ha - cf.c[0][c]);
+  m2[c] = (float)(cf.c[1][c] + hb);
+  m3[c] = (float)(hb - cf.c[1][c]);
+  m4[c] = (float)(cf.c[2][c] + hg);
+  m5[c] = (float)(hg - cf.c[2][c]);
+  m6[c] = (float)cf.c[3][c];
+  m7[c] = (float)cf.c[4][c];
+  m8[c] = (float)cf.c[5][c];
 }
 
 }  // namespace cdiff

@@ -77,7 +77,7 @@ int cdiff_solver::ensure_fp32() {
       if ((rc = alloc32(&q32[a][b]))) return rc;
   if ((rc = alloc32(&base32))) return rc;
   if ((rc = alloc32(&a32))) return rc;
-  dc_newton = env_int("CDIFF_DC_NEWTON", 1) != 0 && kap.t_exp != 0.0 && ndim == 2;
+  dc_newton = env_int("CDIFF_DC_NEWTON", 1) != 0 && kap.t_exp != 0.0 && (ndim == 2 || lin_ok());
   if (dc_newton) {
     if ((rc = alloc32(&ub32))) return rc;
     if ((rc = alloc32(&da32))) return rc;
@@ -115,21 +115,33 @@ int cdiff_solver::ensure_fp32() {
     LAUNCH_CHECK("k_to_f32");
   }
   aux32_valid = true;
-  // coefficient-form sweep: 2D without periodic faces
+  // coefficient-form sweep: no periodic faces
   lin = lin_ok();
-  if (lin) {
+  if (lin && ndim == 2) {
     lin_k = env_int("CDIFF_LIN_K", 4);
     if (lin_k != 2 && lin_k != 4 && lin_k != 6 && lin_k != 8) lin_k = 4;
     const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
     const int rows = lin2d_rows(lin_k);
     lin_chunks = (nj + rows - 1) / rows;
     lin_tiles = ((ni + 63) / 64) * lin_chunks;
+  } else if (lin) {
+    // 3D: one iteration per launch; 64 x 8 columns over chunks of planes
+    lin_k = 1;
+    const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1, nk = g.hi[2] - g.lo[2] + 1;
+    lin3_ch = std::max(1, std::min(nk, env_int("CDIFF_LIN3_CH", 32)));
+    lin3_strips = (ni + 63) / 64;
+    lin_chunks = (nk + lin3_ch - 1) / lin3_ch;
+    lin_tiles = lin3_strips * ((nj + 7) / 8) * lin_chunks;
+  }
+  if (lin) {
     lin_nblocks = (lin_tiles + 7) / 8 * 8;
-    for (int c = 0; c < 5; ++c)
+    const int nm = ndim == 2 ? 5 : 9;
+    for (int c = 0; c < nm; ++c)
       if ((rc = alloc32(&lin_m[c]))) return rc;
     for (int c = 0; c < 3; ++c) {
       if ((rc = alloc32(&lin_fx[c]))) return rc;
       if ((rc = alloc32(&lin_fy[c]))) return rc;
+      if (ndim == 3 && (rc = alloc32(&lin_fz[c]))) return rc;
     }
     if ((rc = alloc32(&lin_cc))) return rc;
     if (lin_nblocks > lin_nparts) {
@@ -140,10 +152,17 @@ int cdiff_solver::ensure_fp32() {
       lin_nparts = lin_nblocks;
     }
     if (!lin_metrics_valid) {
-      hipLaunchKernelGGL(k_lin_metrics, grid_n((int64_t)g.n[0] * g.n[1]), dim3(256), 0, stream, g,
-                         (const double*)coef[0], (const double*)coef[1], (const double*)coef[2],
-                         (const double*)coef[3], (const double*)coef[4], lin_m[0], lin_m[1], lin_m[2], lin_m[3],
-                         lin_m[4]);
+      if (ndim == 2) {
+        hipLaunchKernelGGL(k_lin_metrics, grid_n((int64_t)g.n[0] * g.n[1]), dim3(256), 0, stream, g,
+                           (const double*)coef[0], (const double*)coef[1], (const double*)coef[2],
+                           (const double*)coef[3], (const double*)coef[4], lin_m[0], lin_m[1], lin_m[2], lin_m[3],
+                           lin_m[4]);
+      } else {
+        DcCoefs cf{};
+        for (int t = 0; t < 9; ++t) cf.c[t] = coef[t];
+        hipLaunchKernelGGL(k_lin_metrics3d, grid_n(g.total), dim3(256), 0, stream, g, cf, lin_m[0], lin_m[1],
+                           lin_m[2], lin_m[3], lin_m[4], lin_m[5], lin_m[6], lin_m[7], lin_m[8]);
+      }
       LAUNCH_CHECK("k_lin_metrics");
       lin_metrics_valid = true;
     }
@@ -246,10 +265,11 @@ int cdiff_solver::lin_coefs(double dt, double Vpdtau, int buf) {
   for (int c = 0; c < 3; ++c) {
     out.fx[c] = lin_fx[c];
     out.fy[c] = lin_fy[c];
+    out.fz[c] = lin_fz[c];
   }
   out.cc = lin_cc;
   IterConst ic{L, Vpdtau, dt};
-  hipLaunchKernelGGL(k_lin_coefs, grid_n((int64_t)g.n[0] * g.n[1]), dim3(256), 0, stream, g, gh,
+  hipLaunchKernelGGL(k_lin_coefs, grid_n(g.total), dim3(256), 0, stream, g, gh,
                      (int)(mean == CDIFF_MEAN_HARMONIC), ic, kap.t_exp, (int)dc_newton, (const double*)rho,
                      (const double*)u[buf], (const double*)a64, out);
   LAUNCH_CHECK("k_lin_coefs");
@@ -261,6 +281,35 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit) {
   if (!lin || !lin_coefs_ready) return fail(CDIFF_BAD_ARG, "coefficient-form sweep not set up (run an FP32 step first)");
   if (nit < 1 || nit > lin_k) return fail(CDIFF_BAD_ARG, "iterations per launch out of range");
   const int dst = 1 - src;
+  if (ndim == 3) {
+    Lin3DArgs B{};
+    B.g = g;
+    B.ntiles = lin_tiles;
+    B.chunks = lin_chunks;
+    B.stripsI = lin3_strips;
+    B.ch = lin3_ch;
+    B.inv_dt = (float)(1.0 / dt);
+    B.e_src = u32[src];
+    B.e_dst = u32[dst];
+    for (int a = 0; a < 3; ++a) {
+      B.q_src[a] = q32[a][src];
+      B.q_dst[a] = q32[a][dst];
+    }
+    B.base = base32;
+    B.cc = lin_cc;
+    for (int c = 0; c < 9; ++c) B.m[c] = lin_m[c];
+    for (int c = 0; c < 3; ++c) {
+      B.f[0][c] = lin_fx[c];
+      B.f[1][c] = lin_fy[c];
+      B.f[2][c] = lin_fz[c];
+    }
+    B.part_sum = lin_psum;
+    B.part_max = lin_pmax;
+    B.ctrl = ctrl_in;
+    launch_sweep3d_lin(B, check, lin_nblocks, stream);
+    LAUNCH_CHECK("sweep3d_lin");
+    return CDIFF_OK;
+  }
   Lin2DArgs A{};
   A.g = g;
   A.ntiles = lin_tiles;
@@ -289,8 +338,8 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit) {
 }
 
 bool cdiff_solver::lin_ok() const {
-  if (ndim != 2 || env_int("CDIFF_DC_LIN", 1) == 0) return false;
-  for (int f = 0; f < 4; ++f)
+  if (env_int("CDIFF_DC_LIN", 1) == 0) return false;
+  for (int f = 0; f < 2 * ndim; ++f)
     if (bcs[f].kind == CDIFF_BC_PERIODIC) return false;
   return true;
 }
@@ -349,10 +398,10 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
 // residual of the iterate in u[dstbuf] after u[dstbuf] = u[srcbuf] + e
 int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o) {
   const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  if (lin) return dc_outer_pass(dt, u[srcbuf], e, u[dstbuf], u_prev, iter, o, true);
+  if (lin && ndim == 2) return dc_outer_pass(dt, u[srcbuf], e, u[dstbuf], u_prev, iter, o, true);
   hipLaunchKernelGGL(k_dc_update, grid_n(nfull), dim3(256), 0, stream, g, gh, kap, (const double*)rho,
-                     (const double*)u[srcbuf], e, u[dstbuf], a64, a32, dc_newton ? ub32 : nullptr,
-                     dc_newton ? da32 : nullptr);
+                     (const double*)u[srcbuf], e, u[dstbuf], a64, a32, (dc_newton && !lin) ? ub32 : nullptr,
+                     (dc_newton && !lin) ? da32 : nullptr);
   LAUNCH_CHECK("k_dc_update");
   DcCoefs cf{};
   for (int t = 0; t < 9; ++t) cf.c[t] = coef[t];
@@ -363,6 +412,8 @@ int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf,
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl, (const double*)dc_psum, (const double*)dc_pmax,
                      (int)((nfull + 255) / 256), iter, INT_MAX, o.rtol, o.atol, N_full, o.norm_kind, 1);
   LAUNCH_CHECK("k_finalize");
+  // 3D coefficient form: the sweep coefficients of the new iterate
+  if (lin) return lin_coefs(dt, dc_Vpdtau, dstbuf);
   return CDIFF_OK;
 }
 

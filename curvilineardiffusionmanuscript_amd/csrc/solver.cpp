@@ -350,7 +350,7 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   // learned warm start: features of (T^n, T^{n-1}) and the prediction, before
   // the prologue overwrites u_prev
   warm_pending = false;
-  if (net && o.precision == CDIFF_PREC_FP32_DC && lin_ok()) {
+  if (net && o.precision == CDIFF_PREC_FP32_DC && ndim == 2 && lin_ok()) {
     if (!learn_ts) {
       HIP_TRY(hipMalloc((void**)&learn_ts, 2 * sizeof(double)));
       owned.push_back(learn_ts);

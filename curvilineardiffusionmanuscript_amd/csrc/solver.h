@@ -66,9 +66,11 @@ struct cdiff_solver {
   bool lin = false;
   int lin_k = 4;  // iterations per launch at most
   int lin_tiles = 0, lin_chunks = 0, lin_nblocks = 0;
-  float* lin_m[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  float* lin_m[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   float* lin_fx[3] = {nullptr, nullptr, nullptr};
   float* lin_fy[3] = {nullptr, nullptr, nullptr};
+  float* lin_fz[3] = {nullptr, nullptr, nullptr};
+  int lin3_ch = 0, lin3_strips = 0;  // 3D: planes per chunk, 64-wide strips
   float* lin_cc = nullptr;
   bool lin_metrics_valid = false;
   bool lin_coefs_ready = false;

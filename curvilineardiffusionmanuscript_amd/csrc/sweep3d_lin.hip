@@ -1,0 +1,262 @@
+// sweep3d_lin.hip -- the FP32 correction sweep of the 3D defect-correction
+// step in coefficient form (see sweep2d_lin.hip for the 2D version and the
+// derivation):
+//
+//   face f (xi, eta, zeta) between cells 0 and 1:  q'_f = P_f e_0 - M_f e_1,
+//                                                   q_f <- q'_f + W_f (q_f - q'_f)
+//   cell: D = m0 x_c + m1 x_w + m2 y_c + m3 y_s + m4 z_c + m5 z_s
+//           + cxe (X_eta + Y_xi) + cxz (X_zeta + Z_xi) + cez (Y_zeta + Z_eta)
+//         e <- e + C (r - D - e/dt)
+//
+// with the cross-term groups of flux_divergence.jl:119-239:
+//   X_eta  = (X(0,1,0) + X(-1,1,0)) - (X(0,-1,0) + X(-1,-1,0))
+//   X_zeta = (X(0,0,1) + X(-1,0,1)) - (X(0,0,-1) + X(-1,0,-1))
+//   Y_xi   = (Y(1,0,0) + Y(1,-1,0)) - (Y(-1,0,0) + Y(-1,-1,0))
+//   Y_zeta = (Y(0,0,1) + Y(0,-1,1)) - (Y(0,0,-1) + Y(0,-1,-1))
+//   Z_eta  = (Z(0,1,0) + Z(0,1,-1)) - (Z(0,-1,0) + Z(0,-1,-1))
+//   Z_xi   = (Z(1,0,0) + Z(1,0,-1)) - (Z(-1,0,0) + Z(-1,0,-1))
+//
+// Decomposition: a 512-thread block owns a 64 x 8 column of cells and marches
+// up a chunk of k planes.  Per plane it holds the ringed 66 x 10 plane of e and
+// the xi/eta faces of three consecutive planes and the zeta faces of two in
+// LDS: plane s is updated once the faces of plane s+1 (and the zeta faces
+// between s and s+1) exist.  Every face is computed once per block; the ring
+// of the plane is recomputed by the neighbouring blocks.
+#include "cdiff_dev.h"
+#include "sweep_args.h"
+
+namespace cdiff {
+
+namespace {
+
+constexpr int L3X = 64, L3Y = 8;
+constexpr int R3X = L3X + 2, R3Y = L3Y + 2, R3 = R3X * R3Y;  // ringed plane
+constexpr int L3T = L3X * L3Y;
+constexpr int RING = R3 - L3T;  // 148
+
+// region cell of a thread: slot 0 its owned cell, slot 1 a ring cell (tid < RING)
+__device__ __forceinline__ bool region_cell(int slot, int tid, int& lx, int& ly) {
+  if (slot == 0) {
+    lx = tid % L3X + 1;
+    ly = tid / L3X + 1;
+    return true;
+  }
+  int k = tid;
+  if (k < R3X) {
+    lx = k;
+    ly = 0;
+    return true;
+  }
+  k -= R3X;
+  if (k < R3X) {
+    lx = k;
+    ly = R3Y - 1;
+    return true;
+  }
+  k -= R3X;
+  if (k < L3Y) {
+    lx = 0;
+    ly = 1 + k;
+    return true;
+  }
+  k -= L3Y;
+  if (k < L3Y) {
+    lx = R3X - 1;
+    ly = 1 + k;
+    return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+template <bool CHECK>
+__global__ __launch_bounds__(L3T) void sweep3d_lin_kernel(Lin3DArgs A) {
+  if (A.ctrl->done) return;
+  __shared__ float sE[2][R3], sX[3][R3], sY[3][R3], sZ[2][R3];
+  __shared__ float sPX[CHECK ? 3 : 1][CHECK ? R3 : 1], sPY[CHECK ? 3 : 1][CHECK ? R3 : 1];
+  __shared__ float sPZ[CHECK ? 2 : 1][CHECK ? R3 : 1];
+  __shared__ double sRed[2][L3T / WAVE];
+  const Geo& g = A.g;
+  const int tid = threadIdx.x;
+  const int nb = gridDim.x, per = nb >> 3, b = blockIdx.x;
+  const int tile = (b & 7) * per + (b >> 3);
+  if (tile >= A.ntiles) {
+    if (CHECK && tid == 0) {
+      A.part_sum[b] = 0.0;
+      A.part_max[b] = 0.0;
+    }
+    return;
+  }
+  const int kc = tile % A.chunks;
+  const int rest = tile / A.chunks;
+  const int si = rest % A.stripsI, sj = rest / A.stripsI;
+  const int x0 = g.lo[0] + si * L3X, y0 = g.lo[1] + sj * L3Y;
+  const int z0 = g.lo[2] + kc * A.ch, z1 = min(z0 + A.ch - 1, g.hi[2]);
+  const int gx = x0 - 1, gy = y0 - 1;
+  const float inv_dt = A.inv_dt;
+
+  int rlx[2], rly[2];
+  bool rv[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) rv[p] = region_cell(p, tid, rlx[p], rly[p]);
+  auto in_dom = [&](int i, int j, int k) {
+    return i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1] && k >= g.lo[2] && k <= g.hi[2];
+  };
+  auto face_on = [&](int ax, int i, int j, int k) {
+    const int lo0 = g.lo[0] - (ax == 0), lo1 = g.lo[1] - (ax == 1), lo2 = g.lo[2] - (ax == 2);
+    return i >= lo0 && i <= g.hi[0] && j >= lo1 && j <= g.hi[1] && k >= lo2 && k <= g.hi[2];
+  };
+
+  // e of plane k into sE[k & 1] (0 outside the domain)
+  auto load_e = [&](int k) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      if (!rv[p]) continue;
+      const int i = gx + rlx[p], j = gy + rly[p];
+      sE[k & 1][rly[p] * R3X + rlx[p]] = in_dom(i, j, k) ? A.e_src[at(g, i, j, k)] : 0.f;
+    }
+  };
+  // xi and eta faces of plane k (needs e of plane k), zeta faces between
+  // planes kz and kz+1 (needs e of both); each thread its region cells
+  auto faces_xy = [&](int k, bool last) {
+    const float* E = sE[k & 1];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      if (!rv[p]) continue;
+      const int lx = rlx[p], ly = rly[p], r = ly * R3X + lx;
+      const int i = gx + lx, j = gy + ly;
+      float qx = 0.f, qy = 0.f, px = 0.f, py = 0.f;
+      if (lx < R3X - 1 && face_on(0, i, j, k)) {
+        const int64_t c = at(g, i, j, k);
+        px = A.f[0][0][c] * E[r] - A.f[0][1][c] * E[r + 1];
+        qx = px + A.f[0][2][c] * (A.q_src[0][c] - px);
+      }
+      if (ly < R3Y - 1 && face_on(1, i, j, k)) {
+        const int64_t c = at(g, i, j, k);
+        py = A.f[1][0][c] * E[r] - A.f[1][1][c] * E[r + R3X];
+        qy = py + A.f[1][2][c] * (A.q_src[1][c] - py);
+      }
+      const int s = (k + 3) % 3;
+      sX[s][r] = qx;
+      sY[s][r] = qy;
+      if (CHECK && last) {
+        sPX[s][r] = px;
+        sPY[s][r] = py;
+      }
+    }
+  };
+  auto faces_z = [&](int kz, bool last) {
+    const float* E0 = sE[kz & 1];
+    const float* E1 = sE[(kz + 1) & 1];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      if (!rv[p]) continue;
+      const int lx = rlx[p], ly = rly[p], r = ly * R3X + lx;
+      const int i = gx + lx, j = gy + ly;
+      float qz = 0.f, pz = 0.f;
+      if (face_on(2, i, j, kz)) {
+        const int64_t c = at(g, i, j, kz);
+        pz = A.f[2][0][c] * E0[r] - A.f[2][1][c] * E1[r];
+        qz = pz + A.f[2][2][c] * (A.q_src[2][c] - pz);
+      }
+      sZ[(kz + 2) & 1][r] = qz;
+      if (CHECK && last) sPZ[(kz + 2) & 1][r] = pz;
+    }
+  };
+
+  // ---- prologue: e of planes z0-1, z0; faces of z0-1, z0; zeta faces z0-1
+  load_e(z0 - 1);
+  load_e(z0);
+  __syncthreads();
+  faces_xy(z0 - 1, true);
+  faces_xy(z0, true);
+  faces_z(z0 - 1, true);
+  __syncthreads();
+  load_e(z0 + 1);  // into the buffer of plane z0-1 (no longer read)
+  __syncthreads();
+  faces_xy(z0 + 1, true);
+  faces_z(z0, true);
+  __syncthreads();
+
+  // the owned cell's column
+  const int tx = tid % L3X, ty = tid / L3X;
+  const int i = x0 + tx, j = y0 + ty;
+  const bool col = i <= g.hi[0] && j <= g.hi[1];
+  const int r0 = (ty + 1) * R3X + (tx + 1);
+  double sq = 0.0, smax = 0.0;
+  for (int k = z0; k <= z1; ++k) {
+    // ---- update of plane k (faces of k-1..k+1, zeta faces k-1, k exist)
+    const int sm = (k + 2) % 3, sc = (k + 3) % 3, sp = (k + 4) % 3;
+    const int zc = (k + 2) & 1, zm = (k + 1) & 1;
+    if (col) {
+      const int64_t c = at(g, i, j, k);
+      auto div = [&](const float* Xm, const float* Xc, const float* Xp, const float* Ym, const float* Yc,
+                     const float* Yp, const float* Zm, const float* Zc) {
+        const float xc = Xc[r0], xw = Xc[r0 - 1], yc = Yc[r0], ys = Yc[r0 - R3X], zcv = Zc[r0], zs = Zm[r0];
+        const float x_eta = (Xc[r0 + R3X] + Xc[r0 + R3X - 1]) - (Xc[r0 - R3X] + Xc[r0 - R3X - 1]);
+        const float x_zeta = (Xp[r0] + Xp[r0 - 1]) - (Xm[r0] + Xm[r0 - 1]);
+        const float y_xi = (Yc[r0 + 1] + Yc[r0 - R3X + 1]) - (Yc[r0 - 1] + Yc[r0 - R3X - 1]);
+        const float y_zeta = (Yp[r0] + Yp[r0 - R3X]) - (Ym[r0] + Ym[r0 - R3X]);
+        const float z_eta = (Zc[r0 + R3X] + Zm[r0 + R3X]) - (Zc[r0 - R3X] + Zm[r0 - R3X]);
+        const float z_xi = (Zc[r0 + 1] + Zm[r0 + 1]) - (Zc[r0 - 1] + Zm[r0 - 1]);
+        return A.m[0][c] * xc + A.m[1][c] * xw + A.m[2][c] * yc + A.m[3][c] * ys + A.m[4][c] * zcv +
+               A.m[5][c] * zs + A.m[6][c] * (x_eta + y_xi) + A.m[7][c] * (x_zeta + z_xi) +
+               A.m[8][c] * (y_zeta + z_eta);
+      };
+      const float D = div(sX[sm], sX[sc], sX[sp], sY[sm], sY[sc], sY[sp], sZ[zm], sZ[zc]);
+      const float e = sE[k & 1][r0];
+      const float rb = A.base[c];
+      const float en = e + A.cc[c] * (rb - D - e * inv_dt);
+      A.e_dst[c] = en;
+      A.q_dst[0][c] = sX[sc][r0];
+      A.q_dst[1][c] = sY[sc][r0];
+      A.q_dst[2][c] = sZ[zc][r0];
+      if (CHECK) {
+        const float Dp = div(sPX[sm], sPX[sc], sPX[sp], sPY[sm], sPY[sc], sPY[sp], sPZ[zm], sPZ[zc]);
+        const double res = (double)(rb - en * inv_dt - Dp);
+        sq += res * res;
+        smax = nanmax(smax, fabs(res));
+      }
+      // boundary faces below the domain that no cell owns
+      if (i == g.lo[0]) A.q_dst[0][at(g, i - 1, j, k)] = sX[sc][r0 - 1];
+      if (j == g.lo[1]) A.q_dst[1][at(g, i, j - 1, k)] = sY[sc][r0 - R3X];
+      if (k == g.lo[2]) A.q_dst[2][at(g, i, j, k - 1)] = sZ[zm][r0];
+    }
+    if (k == z1) break;
+    __syncthreads();  // plane k's faces and e are read
+    // ---- advance: e of plane k+2, faces of plane k+2, zeta faces k+1
+    load_e(k + 2);
+    __syncthreads();
+    faces_xy(k + 2, true);
+    faces_z(k + 1, true);
+    __syncthreads();
+  }
+  if (CHECK) {
+    sq = wave_sum(sq);
+    smax = wave_max(smax);
+    if ((tid & (WAVE - 1)) == 0) {
+      sRed[0][tid / WAVE] = sq;
+      sRed[1][tid / WAVE] = smax;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0, m = 0.0;
+      for (int w = 0; w < L3T / WAVE; ++w) {
+        s += sRed[0][w];
+        m = nanmax(m, sRed[1][w]);
+      }
+      A.part_sum[b] = s;
+      A.part_max[b] = m;
+    }
+  }
+}
+
+void launch_sweep3d_lin(const Lin3DArgs& A, bool check, int nblocks, hipStream_t s) {
+  if (check)
+    hipLaunchKernelGGL(sweep3d_lin_kernel<true>, dim3(nblocks), dim3(L3T), 0, s, A);
+  else
+    hipLaunchKernelGGL(sweep3d_lin_kernel<false>, dim3(nblocks), dim3(L3T), 0, s, A);
+}
+
+}  // namespace cdiff

@@ -126,4 +126,24 @@ struct Lin2DArgs {
 int lin2d_rows(int kmax);
 void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, bool check, int nblocks, hipStream_t s);
 
+// FP32 correction sweep in coefficient form, 3D (sweep3d_lin.hip): 64 x 8
+// columns marching over chunks of `ch` planes.
+struct Lin3DArgs {
+  Geo g;
+  int ntiles, chunks, stripsI, ch;
+  float inv_dt;
+  const float* __restrict__ e_src;
+  float* __restrict__ e_dst;
+  const float* q_src[3];
+  float* q_dst[3];
+  const float* __restrict__ base;
+  const float* __restrict__ cc;
+  const float* m[9];     // m0..m5 (x_c, x_w, y_c, y_s, z_c, z_s), cxe, cxz, cez
+  const float* f[3][3];  // [axis][P, M, W]
+  double* part_sum;
+  double* part_max;
+  const Ctrl* ctrl;
+};
+void launch_sweep3d_lin(const Lin3DArgs& A, bool check, int nblocks, hipStream_t s);
+
 }  // namespace cdiff

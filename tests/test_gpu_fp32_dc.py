@@ -34,7 +34,8 @@ def _oracle_residual_rms(mesh, bcs_o, kappa, T_new, T_old, dt):
 
 @pytest.mark.parametrize(
     "case,lin",
-    [("config1", "1"), ("config2", "1"), ("config1", "0"), ("config2", "0"), ("config4", "1")],
+    [("config1", "1"), ("config2", "1"), ("config1", "0"), ("config2", "0"), ("config4", "1"), ("config4", "0"),
+     ("config4d", "1")],
 )
 def test_dc_reaches_fp64_tolerance(case, lin, monkeypatch):
     # lin = 1: the 2D correction sweep in coefficient form (several iterations
@@ -49,9 +50,20 @@ def test_dc_reaches_fp64_tolerance(case, lin, monkeypatch):
         bd = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
         bo = [("dirichlet", 1.0)] + [("neumann", 0)] * 3
         kappa, dt, nsteps = (1.0, 2.5, 0.0), 1e-5, 3
-    else:
+    elif case == "config4":
         mesh, T0 = gaussian_spots_3d(20, seed=1)
         bd, bo, kappa, dt, nsteps = [S.NeumannBC()] * 6, [("neumann", 0)] * 6, (1.0, 3.0, 0.0), 1e-4, 2
+    else:
+        # 3D with fixed-value faces (one ghost layer) on three sides, ragged tiles
+        from curvilineardiffusionmanuscript_amd.mesh import CurvilinearGrid3D, wavy_map_3d
+
+        mesh = CurvilinearGrid3D.from_mapping(wavy_map_3d(0.04), (70, 13, 11), 1)
+        x, y, z = mesh.centroid_coordinates.x, mesh.centroid_coordinates.y, mesh.centroid_coordinates.z
+        T0 = np.asfortranarray(0.3 + np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2) / 0.02))
+        bd = [S.DirichletBC(0.5), S.NeumannBC(), S.NeumannBC(), S.DirichletBC(0.2), S.NeumannBC(), S.DirichletBC(1.0)]
+        bo = [("dirichlet", 0.5), ("neumann", 0), ("neumann", 0), ("dirichlet", 0.2), ("neumann", 0),
+              ("dirichlet", 1.0)]
+        kappa, dt, nsteps = (1.0, 2.0, 0.0), 2e-4, 2
     law = S.PowerLawConductivity(*kappa)
     dev = S.PseudoTransientSolver(mesh, bd)
     ora = O.OracleSolver(mesh, bo, kappa=kappa)
