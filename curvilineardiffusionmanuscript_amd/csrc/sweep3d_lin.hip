@@ -108,84 +108,121 @@ __global__ __launch_bounds__(L3T) void sweep3d_lin_kernel(Lin3DArgs A) {
     return i >= lo0 && i <= g.hi[0] && j >= lo1 && j <= g.hi[1] && k >= lo2 && k <= g.hi[2];
   };
 
-  // e of plane k into sE[k & 1] (0 outside the domain)
-  auto load_e = [&](int k) {
+  // Everything a plane step needs from memory, fetched one step ahead so the
+  // loads are in flight while the previous plane is updated: e of plane k,
+  // the xi/eta faces of plane k and the zeta face below it (between planes
+  // k-1 and k, stored at plane k-1): P, M, W and the old flux.
+  struct Fetch {
+    float e, f[3][4];
+  };
+  auto fetch = [&](int k, Fetch* F, bool zf = true) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
+      F[p].e = 0.f;
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) F[p].f[ax][0] = F[p].f[ax][1] = F[p].f[ax][2] = F[p].f[ax][3] = 0.f;
       if (!rv[p]) continue;
       const int i = gx + rlx[p], j = gy + rly[p];
-      sE[k & 1][rly[p] * R3X + rlx[p]] = in_dom(i, j, k) ? A.e_src[at(g, i, j, k)] : 0.f;
+      if (in_dom(i, j, k)) F[p].e = A.e_src[at(g, i, j, k)];
+      if (rlx[p] < R3X - 1 && face_on(0, i, j, k)) {
+        const int64_t c = at(g, i, j, k);
+        F[p].f[0][0] = A.f[0][0][c];
+        F[p].f[0][1] = A.f[0][1][c];
+        F[p].f[0][2] = A.f[0][2][c];
+        F[p].f[0][3] = A.q_src[0][c];
+      }
+      if (rly[p] < R3Y - 1 && face_on(1, i, j, k)) {
+        const int64_t c = at(g, i, j, k);
+        F[p].f[1][0] = A.f[1][0][c];
+        F[p].f[1][1] = A.f[1][1][c];
+        F[p].f[1][2] = A.f[1][2][c];
+        F[p].f[1][3] = A.q_src[1][c];
+      }
+      if (zf && face_on(2, i, j, k - 1)) {
+        const int64_t c = at(g, i, j, k - 1);
+        F[p].f[2][0] = A.f[2][0][c];
+        F[p].f[2][1] = A.f[2][1][c];
+        F[p].f[2][2] = A.f[2][2][c];
+        F[p].f[2][3] = A.q_src[2][c];
+      }
     }
   };
-  // xi and eta faces of plane k (needs e of plane k), zeta faces between
-  // planes kz and kz+1 (needs e of both); each thread its region cells
-  auto faces_xy = [&](int k, bool last) {
+  // plane step k: e of plane k into LDS; then the xi/eta faces of plane k and
+  // the zeta faces between planes k-1 and k (e of k-1 is in the other buffer)
+  auto store_e = [&](int k, const Fetch* F) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+      if (rv[p]) sE[k & 1][rly[p] * R3X + rlx[p]] = F[p].e;
+  };
+  auto faces = [&](int k, const Fetch* F, bool zf = true) {
     const float* E = sE[k & 1];
+    const float* Em = sE[(k + 1) & 1];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       if (!rv[p]) continue;
-      const int lx = rlx[p], ly = rly[p], r = ly * R3X + lx;
-      const int i = gx + lx, j = gy + ly;
-      float qx = 0.f, qy = 0.f, px = 0.f, py = 0.f;
-      if (lx < R3X - 1 && face_on(0, i, j, k)) {
-        const int64_t c = at(g, i, j, k);
-        px = A.f[0][0][c] * E[r] - A.f[0][1][c] * E[r + 1];
-        qx = px + A.f[0][2][c] * (A.q_src[0][c] - px);
-      }
-      if (ly < R3Y - 1 && face_on(1, i, j, k)) {
-        const int64_t c = at(g, i, j, k);
-        py = A.f[1][0][c] * E[r] - A.f[1][1][c] * E[r + R3X];
-        qy = py + A.f[1][2][c] * (A.q_src[1][c] - py);
-      }
-      const int s = (k + 3) % 3;
-      sX[s][r] = qx;
-      sY[s][r] = qy;
-      if (CHECK && last) {
+      const int r = rly[p] * R3X + rlx[p];
+      const float ec = E[r];
+      const float px = F[p].f[0][0] * ec - F[p].f[0][1] * (rlx[p] < R3X - 1 ? E[r + 1] : 0.f);
+      const float py = F[p].f[1][0] * ec - F[p].f[1][1] * (rly[p] < R3Y - 1 ? E[r + R3X] : 0.f);
+      const int s = (k + 3) % 3, z = (k + 1) & 1;  // slots of plane k / of the zeta face k-1
+      sX[s][r] = px + F[p].f[0][2] * (F[p].f[0][3] - px);
+      sY[s][r] = py + F[p].f[1][2] * (F[p].f[1][3] - py);
+      if (CHECK) {
         sPX[s][r] = px;
         sPY[s][r] = py;
       }
-    }
-  };
-  auto faces_z = [&](int kz, bool last) {
-    const float* E0 = sE[kz & 1];
-    const float* E1 = sE[(kz + 1) & 1];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      if (!rv[p]) continue;
-      const int lx = rlx[p], ly = rly[p], r = ly * R3X + lx;
-      const int i = gx + lx, j = gy + ly;
-      float qz = 0.f, pz = 0.f;
-      if (face_on(2, i, j, kz)) {
-        const int64_t c = at(g, i, j, kz);
-        pz = A.f[2][0][c] * E0[r] - A.f[2][1][c] * E1[r];
-        qz = pz + A.f[2][2][c] * (A.q_src[2][c] - pz);
+      if (zf) {
+        const float pz = F[p].f[2][0] * Em[r] - F[p].f[2][1] * ec;
+        sZ[z][r] = pz + F[p].f[2][2] * (F[p].f[2][3] - pz);
+        if (CHECK) sPZ[z][r] = pz;
       }
-      sZ[(kz + 2) & 1][r] = qz;
-      if (CHECK && last) sPZ[(kz + 2) & 1][r] = pz;
     }
   };
-
-  // ---- prologue: e of planes z0-1, z0; faces of z0-1, z0; zeta faces z0-1
-  load_e(z0 - 1);
-  load_e(z0);
-  __syncthreads();
-  faces_xy(z0 - 1, true);
-  faces_xy(z0, true);
-  faces_z(z0 - 1, true);
-  __syncthreads();
-  load_e(z0 + 1);  // into the buffer of plane z0-1 (no longer read)
-  __syncthreads();
-  faces_xy(z0 + 1, true);
-  faces_z(z0, true);
-  __syncthreads();
-
-  // the owned cell's column
+  // per owned cell: the update data of plane k
   const int tx = tid % L3X, ty = tid / L3X;
   const int i = x0 + tx, j = y0 + ty;
   const bool col = i <= g.hi[0] && j <= g.hi[1];
   const int r0 = (ty + 1) * R3X + (tx + 1);
+  struct Upd {
+    float rb, cc, m[9];
+  };
+  auto fetch_upd = [&](int k, Upd& U) {
+    if (!col || k > z1) return;
+    const int64_t c = at(g, i, j, k);
+    U.rb = A.base[c];
+    U.cc = A.cc[c];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) U.m[q] = A.m[q][c];
+  };
+
+  // ---- prologue: plane steps z0-1, z0, z0+1 (the zeta faces below z0-1 are
+  //      not needed)
+  Fetch F[2];
+  fetch(z0 - 1, F, false);
+  store_e(z0 - 1, F);
+  __syncthreads();
+  faces(z0 - 1, F, false);
+  fetch(z0, F);
+  __syncthreads();
+  store_e(z0, F);
+  __syncthreads();
+  faces(z0, F);
+  fetch(z0 + 1, F);
+  Upd U{}, Un{};
+  fetch_upd(z0, U);
+  __syncthreads();
+  store_e(z0 + 1, F);
+  __syncthreads();
+  faces(z0 + 1, F);
+  __syncthreads();
+
   double sq = 0.0, smax = 0.0;
   for (int k = z0; k <= z1; ++k) {
+    // loads of the next plane step and of the next update, in flight during this update
+    if (k < z1) {
+      fetch(k + 2, F);
+      fetch_upd(k + 1, Un);
+    }
     // ---- update of plane k (faces of k-1..k+1, zeta faces k-1, k exist)
     const int sm = (k + 2) % 3, sc = (k + 3) % 3, sp = (k + 4) % 3;
     const int zc = (k + 2) & 1, zm = (k + 1) & 1;
@@ -200,21 +237,19 @@ __global__ __launch_bounds__(L3T) void sweep3d_lin_kernel(Lin3DArgs A) {
         const float y_zeta = (Yp[r0] + Yp[r0 - R3X]) - (Ym[r0] + Ym[r0 - R3X]);
         const float z_eta = (Zc[r0 + R3X] + Zm[r0 + R3X]) - (Zc[r0 - R3X] + Zm[r0 - R3X]);
         const float z_xi = (Zc[r0 + 1] + Zm[r0 + 1]) - (Zc[r0 - 1] + Zm[r0 - 1]);
-        return A.m[0][c] * xc + A.m[1][c] * xw + A.m[2][c] * yc + A.m[3][c] * ys + A.m[4][c] * zcv +
-               A.m[5][c] * zs + A.m[6][c] * (x_eta + y_xi) + A.m[7][c] * (x_zeta + z_xi) +
-               A.m[8][c] * (y_zeta + z_eta);
+        return U.m[0] * xc + U.m[1] * xw + U.m[2] * yc + U.m[3] * ys + U.m[4] * zcv + U.m[5] * zs +
+               U.m[6] * (x_eta + y_xi) + U.m[7] * (x_zeta + z_xi) + U.m[8] * (y_zeta + z_eta);
       };
       const float D = div(sX[sm], sX[sc], sX[sp], sY[sm], sY[sc], sY[sp], sZ[zm], sZ[zc]);
       const float e = sE[k & 1][r0];
-      const float rb = A.base[c];
-      const float en = e + A.cc[c] * (rb - D - e * inv_dt);
+      const float en = e + U.cc * (U.rb - D - e * inv_dt);
       A.e_dst[c] = en;
       A.q_dst[0][c] = sX[sc][r0];
       A.q_dst[1][c] = sY[sc][r0];
       A.q_dst[2][c] = sZ[zc][r0];
       if (CHECK) {
         const float Dp = div(sPX[sm], sPX[sc], sPX[sp], sPY[sm], sPY[sc], sPY[sp], sPZ[zm], sPZ[zc]);
-        const double res = (double)(rb - en * inv_dt - Dp);
+        const double res = (double)(U.rb - en * inv_dt - Dp);
         sq += res * res;
         smax = nanmax(smax, fabs(res));
       }
@@ -224,12 +259,12 @@ __global__ __launch_bounds__(L3T) void sweep3d_lin_kernel(Lin3DArgs A) {
       if (k == g.lo[2]) A.q_dst[2][at(g, i, j, k - 1)] = sZ[zm][r0];
     }
     if (k == z1) break;
+    U = Un;
     __syncthreads();  // plane k's faces and e are read
-    // ---- advance: e of plane k+2, faces of plane k+2, zeta faces k+1
-    load_e(k + 2);
+    // ---- plane step k+2
+    store_e(k + 2, F);
     __syncthreads();
-    faces_xy(k + 2, true);
-    faces_z(k + 1, true);
+    faces(k + 2, F);
     __syncthreads();
   }
   if (CHECK) {
