@@ -52,6 +52,30 @@ struct DcOuter2D {
   double* pmax;
 };
 __global__ void k_dc_outer2d(DcOuter2D A);
+// Fused outer pass of the 3D step (dc_outer3d.hip): the same work over 64 x 8
+// columns marching up chunks of `ch` planes.
+struct DcOuter3D {
+  Geo g;
+  Ghost gh;
+  KappaLaw kap;
+  IterConst ic;
+  int harmonic, newton, ntiles, chunks, stripsI, ch;
+  double t_exp;
+  const double* rho;
+  const double* u_old;
+  const float* e;  // nullable: no correction
+  double* u_new;
+  double* a64;
+  const double* uprev;
+  const double* c[9];  // g11 g22 g33 g12 g13 g23 ac bc gc
+  const double *S, *tzA, *tzB;
+  float* r32;
+  float* f[3][3];  // P, M, W per axis
+  float* cc;
+  double* psum;
+  double* pmax;
+};
+__global__ void k_dc_outer3d(DcOuter3D A);
 __global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double t_exp, int newton, const double* rho,
                             const double* u, const double* a, LinOut out);
 __global__ void k_to_f32(int64_t n, const double* src, float* dst);

@@ -1,0 +1,399 @@
+// dc_outer3d.hip -- one pass per outer iteration of the 3D defect-correction
+// step: apply the correction, evaluate the FP64 residual, and build the
+// coefficients of the next FP32 correction sweep (sweep3d_lin.hip).
+//
+// A 512-thread block owns a 64 x 8 column of cells and marches up a chunk of
+// planes z0..z1.  Plane step p, for p = z0-1 .. z1+1:
+//   cells    of the ringed 66 x 10 plane p: u = u_old + e (domain), the BC
+//            rule for first-layer ghosts (local rules only: no periodic
+//            faces), alpha(u), theta, dtau, d(alpha)/du (FP64, LDS, planes p-1
+//            and p); writes u, alpha and the sweep's C of the column
+//   faces    q' = -abar (u1 - u0) on expand_lower(domain, axis) (FP64, LDS)
+//            of the xi and eta faces of plane p and of the zeta faces between
+//            planes p-1 and p; the sweep coefficients P, M, W of the column's
+//            faces (k_lin_coefs)
+//   residual of plane p-1 per column cell, in FP64: the reference residual
+//            (residuals.jl:24-47) with the 3D divergence of
+//            flux_divergence.jl:119-239, summed in the order of k_dc_residual;
+//            the faces of planes p-2 and p-1 it needs are carried in registers
+//            from the previous steps, so the face planes are single-buffered.
+// The cell inputs of the next plane step are loaded one step ahead, the
+// residual inputs at the start of the step that uses them.
+#include <limits>
+
+#include "column3d.h"
+#include "dc.h"
+
+namespace cdiff {
+
+namespace {
+
+constexpr int FXN = L3X + 1;  // xi faces per ring row: between region columns fx and fx+1
+constexpr int FYN = L3Y + 1;  // eta face rows: between region rows fy and fy+1
+constexpr int NFX = FXN * R3Y, NFY = R3X * FYN;
+constexpr int NFACE = NFX + NFY + R3;
+
+__device__ __forceinline__ double gfac3(const Ghost& gh, int f) {
+  const int m = gh.mode[f];
+  return m == G_COPY ? 1.0 : (m == G_NEG ? -1.0 : 0.0);
+}
+
+__device__ __forceinline__ double dadu3(double t_exp, double a, double u) {
+  const double d = (u != 0.0 && isfinite(u)) ? t_exp * a / u : 0.0;
+  return isfinite(d) ? d : 0.0;
+}
+
+// gh.mode[f], gh.src[f], gh.val[f] for a face index known only at run time,
+// by selection (an indexed read would place the kernel argument in scratch)
+template <typename T>
+__device__ __forceinline__ T pick6(const T (&v)[6], int f) {
+  T r = v[0];
+#pragma unroll
+  for (int q = 1; q < 6; ++q) r = f == q ? v[q] : r;
+  return r;
+}
+
+// what a region cell needs from memory
+struct CellIn {
+  double v;   // u_old of the cell, or of the BC source cell of a face ghost
+  double rho;
+  float e;    // correction of the same cell
+  int kind;   // 0 none, 1 domain cell or copied face ghost, -1 negated face ghost, 2 fixed value, 3 u_old
+};
+
+}  // namespace
+
+__global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
+  __shared__ double sU[2][R3], sA[2][R3], sT[2][R3], sDa[2][R3];
+  __shared__ double sX[NFX], sY[NFY], sZ[R3];
+  __shared__ double sRed[2][L3T / WAVE];
+  const Geo& g = A.g;
+  const int tid = threadIdx.x;
+  const int b = blockIdx.x;
+  Column3D col3;
+  if (!column_of(g, A.ntiles, A.chunks, A.stripsI, A.ch, col3)) {
+    if (tid == 0) {
+      A.psum[b] = 0.0;
+      A.pmax[b] = 0.0;
+    }
+    return;
+  }
+  const int x0 = col3.x0, y0 = col3.y0, z0 = col3.z0, z1 = col3.z1;
+  const int gx = x0 - 1, gy = y0 - 1;
+  const int harm = A.harmonic;
+  const double dt = A.ic.dt;
+
+  int rlx[2], rly[2];
+  bool rv[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) rv[s] = region_cell(s, tid, rlx[s], rly[s]);
+
+  // ---- loads of the cells of plane p
+  auto fetch_cells = [&](int p, CellIn* C) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      C[s].v = 0.0;
+      C[s].rho = 1.0;
+      C[s].e = 0.f;
+      C[s].kind = 0;
+      if (!rv[s]) continue;
+      const int i = gx + rlx[s], j = gy + rly[s];
+      const bool valid = i >= g.lo[0] - 1 && i <= g.hi[0] + 1 && j >= g.lo[1] - 1 && j <= g.hi[1] + 1 &&
+                         p >= g.lo[2] - 1 && p <= g.hi[2] + 1;
+      if (!valid) continue;
+      const bool oi = i < g.lo[0] || i > g.hi[0], oj = j < g.lo[1] || j > g.hi[1], ok = p < g.lo[2] || p > g.hi[2];
+      const int nout = (int)oi + (int)oj + (int)ok;
+      const int64_t c = at(g, i, j, p);
+      if (A.rho) C[s].rho = A.rho[c];
+      if (nout == 0) {
+        C[s].kind = 1;
+        C[s].v = A.u_old[c];
+        if (A.e) C[s].e = A.e[c];
+      } else if (nout == 1) {
+        const int f = oi ? (i < g.lo[0] ? 0 : 1) : (oj ? (j < g.lo[1] ? 2 : 3) : (p < g.lo[2] ? 4 : 5));
+        const int mode = pick6(A.gh.mode, f);
+        if (mode == G_CONST) {
+          C[s].kind = 2;
+          C[s].v = pick6(A.gh.val, f);
+        } else {
+          const int src = pick6(A.gh.src, f);
+          const int64_t sc = at(g, oi ? src : i, oj ? src : j, ok ? src : p);
+          C[s].kind = mode == G_NEG ? -1 : 1;
+          C[s].v = A.u_old[sc];
+          if (A.e) C[s].e = A.e[sc];
+        }
+      } else {
+        C[s].kind = 3;
+        C[s].v = A.u_old[c];
+      }
+    }
+  };
+
+  // ---- cells of plane p from their loads: LDS, and the column's writes
+  double u_own = 0.0, a_own = 0.0;  // column cell of plane p (slot 0)
+  auto cells = [&](int p, const CellIn* C) {
+    const int buf = p & 1;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (!rv[s]) continue;
+      const int r = rly[s] * R3X + rlx[s];
+      double u = 0.0, a = 0.0, th = 0.0, da = 0.0;
+      if (C[s].kind != 0) {
+        if (C[s].kind == 1 || C[s].kind == -1) {
+          const double us = C[s].v + (double)C[s].e;
+          u = C[s].kind == -1 ? -us : us;
+        } else {
+          u = C[s].v;
+        }
+        a = alpha_of<double>(A.kap, C[s].rho, u);
+        double dtau;
+        iter_params<double>(A.ic, C[s].rho, a, dtau, th);
+        da = dadu3(A.t_exp, a, u);
+        const int i = gx + rlx[s], j = gy + rly[s];
+        const bool gi = i < g.lo[0] || i > g.hi[0], gj = j < g.lo[1] || j > g.hi[1];
+        const bool gk = p < g.lo[2] || p > g.hi[2];
+        const bool own_i = rlx[s] >= 1 && rlx[s] <= L3X, own_j = rly[s] >= 1 && rly[s] <= L3Y;
+        const bool own_k = p >= z0 && p <= z1;
+        const int64_t c = at(g, i, j, p);
+        if (!gi && !gj && !gk) {
+          if (own_i && own_j && own_k) {
+            A.u_new[c] = u;
+            A.a64[c] = a;
+            A.cc[c] = (float)(dtau / (1.0 + dtau / dt));
+          }
+        } else if ((int)gi + (int)gj + (int)gk == 1) {
+          // face ghosts: the block owning the adjacent column cells writes them
+          if ((gi && own_j && own_k) || (gj && own_i && own_k) || (gk && own_i && own_j)) {
+            A.u_new[c] = u;
+            A.a64[c] = a;
+          }
+        }
+      }
+      sU[buf][r] = u;
+      sA[buf][r] = a;
+      sT[buf][r] = th;
+      sDa[buf][r] = da;
+      if (s == 0) {
+        u_own = u;
+        a_own = a;
+      }
+    }
+  };
+
+  // ---- faces of plane p (zeta faces between p-1 and p when `zf`)
+  auto coefs = [&](int b0, int r0, int b1, int r1, int flo, int fhi, bool lo_face, bool hi_face, float* F0, float* F1,
+                   float* F2, int64_t gc) {
+    const double a0 = sA[b0][r0], a1 = sA[b1][r1];
+    const double abar = face_mean<double>(harm, a0, a1);
+    const double tbar = face_mean<double>(harm, sT[b0][r0], sT[b1][r1]);
+    double P = abar, M = abar;
+    if (A.newton) {
+      const double sum = a0 + a1;
+      double d0 = 0.5, d1 = 0.5;
+      if (harm) {
+        d0 = (sum != 0.0) ? 2.0 * a1 * a1 / (sum * sum) : 0.0;
+        d1 = (sum != 0.0) ? 2.0 * a0 * a0 / (sum * sum) : 0.0;
+      }
+      const double du = sU[b1][r1] - sU[b0][r0];
+      P -= d0 * sDa[b0][r0] * du;
+      M += d1 * sDa[b1][r1] * du;
+    }
+    if (lo_face) {
+      M -= gfac3(A.gh, flo) * P;
+      P = 0.0;
+    } else if (hi_face) {
+      P -= gfac3(A.gh, fhi) * M;
+      M = 0.0;
+    }
+    F0[gc] = (float)P;
+    F1[gc] = (float)M;
+    F2[gc] = (float)(tbar / (1.0 + tbar));
+  };
+  auto faces = [&](int p, bool zf) {
+    const int bc = p & 1, bm = (p + 1) & 1;
+    const bool pdom = p >= g.lo[2] && p <= g.hi[2];
+    const bool own_p = p >= z0 && p <= z1;
+#pragma unroll 1
+    for (int q = 0; q < (NFACE + L3T - 1) / L3T; ++q) {
+      const int t = tid + q * L3T;
+      if (t < NFX) {
+        const int fx = t % FXN, cy = t / FXN;
+        const int i = gx + fx, j = gy + cy;
+        double v = 0.0;
+        if (pdom && i >= g.lo[0] - 1 && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1]) {
+          const int r = cy * R3X + fx;
+          v = -face_mean<double>(harm, sA[bc][r], sA[bc][r + 1]) * (sU[bc][r + 1] - sU[bc][r]);
+          if (own_p && cy >= 1 && cy <= L3Y && (fx >= 1 || i == g.lo[0] - 1))
+            coefs(bc, r, bc, r + 1, 0, 1, i == g.lo[0] - 1, i == g.hi[0], A.f[0][0], A.f[0][1], A.f[0][2], at(g, i, j, p));
+        }
+        sX[t] = v;
+      } else if (t < NFX + NFY) {
+        const int u = t - NFX;
+        const int fx = u % R3X, fy = u / R3X;
+        const int i = gx + fx, j = gy + fy;
+        double v = 0.0;
+        if (pdom && i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] - 1 && j <= g.hi[1]) {
+          const int r = fy * R3X + fx;
+          v = -face_mean<double>(harm, sA[bc][r], sA[bc][r + R3X]) * (sU[bc][r + R3X] - sU[bc][r]);
+          if (own_p && fx >= 1 && fx <= L3X && (fy >= 1 || j == g.lo[1] - 1))
+            coefs(bc, r, bc, r + R3X, 2, 3, j == g.lo[1] - 1, j == g.hi[1], A.f[1][0], A.f[1][1], A.f[1][2], at(g, i, j, p));
+        }
+        sY[u] = v;
+      } else if (t < NFACE) {
+        const int r = t - NFX - NFY;
+        const int lx = r % R3X, ly = r / R3X;
+        const int i = gx + lx, j = gy + ly, k = p - 1;  // face between planes k and k+1, stored at k
+        double v = 0.0;
+        if (zf && i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1] && k >= g.lo[2] - 1 &&
+            k <= g.hi[2]) {
+          v = -face_mean<double>(harm, sA[bm][r], sA[bc][r]) * (sU[bc][r] - sU[bm][r]);
+          const bool own_k = (k >= z0 && k <= z1) || k == g.lo[2] - 1;
+          if (own_k && lx >= 1 && lx <= L3X && ly >= 1 && ly <= L3Y)
+            coefs(bm, r, bc, r, 4, 5, k == g.lo[2] - 1, k == g.hi[2], A.f[2][0], A.f[2][1], A.f[2][2], at(g, i, j, k));
+        }
+        sZ[r] = v;
+      }
+    }
+  };
+
+  // ---- per column cell: residual inputs of plane p
+  const int tx = tid % L3X, ty = tid / L3X;
+  const int ci = x0 + tx, cj = y0 + ty;
+  const bool col = ci <= g.hi[0] && cj <= g.hi[1];
+  struct ResIn {
+    double c[9], uprev, src;
+  };
+  auto fetch_res = [&](int p, ResIn& R) {
+    if (!col || p < z0 || p > z1) return;
+    const int64_t c = at(g, ci, cj, p);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) R.c[q] = A.c[q][c];
+    R.uprev = A.uprev[c];
+    R.src = A.S ? A.S[c] : 0.0;
+  };
+
+  // faces around the column cell, carried between plane steps
+  struct Plane {
+    double xc, xw, xeta, yc, ys, yxi;  // xi / eta faces of the plane
+    double xsum, ysum;                 // X(0) + X(-1), Y(0) + Y(0,-1)
+  };
+  struct ZSt {
+    double c, e, w, n, s;  // zeta faces stored at a plane: (0,0), (+1,0), (-1,0), (0,+1), (0,-1)
+  };
+  const int x_r = (ty + 1) * FXN + tx + 1;  // X(0,0) of the column cell
+  const int y_r = (ty + 1) * R3X + tx + 1;  // Y(0,0) and Z(0,0)
+  auto read_plane = [&](Plane& P) {
+    P.xc = sX[x_r];
+    P.xw = sX[x_r - 1];
+    P.xeta = (sX[x_r + FXN] + sX[x_r + FXN - 1]) - (sX[x_r - FXN] + sX[x_r - FXN - 1]);
+    P.yc = sY[y_r];
+    P.ys = sY[y_r - R3X];
+    P.yxi = (sY[y_r + 1] + sY[y_r - R3X + 1]) - (sY[y_r - 1] + sY[y_r - R3X - 1]);
+    P.xsum = P.xc + P.xw;
+    P.ysum = P.yc + P.ys;
+  };
+  auto read_z = [&](ZSt& Z) {
+    Z.c = sZ[y_r];
+    Z.e = sZ[y_r + 1];
+    Z.w = sZ[y_r - 1];
+    Z.n = sZ[y_r + R3X];
+    Z.s = sZ[y_r - R3X];
+  };
+
+  double sq = 0.0, mx = 0.0;
+  // residual of plane k from the faces of planes k-1 (sums xm, ym), k (pc),
+  // k+1 (pn) and the zeta faces stored at k-1 (zm) and k (zc)
+  auto residual = [&](int k, const ResIn& R, double xm, double ym, const Plane& pc, const Plane& pn, const ZSt& zm,
+                      const ZSt& zc, double un, double an) {
+    if (!col) return;
+    const double a1 = R.c[0] * (pc.xc - pc.xw);
+    const double a2 = R.c[1] * (pc.yc - pc.ys);
+    const double a3 = R.c[2] * (zc.c - zm.c);
+    const double qi_eta = R.c[3] * pc.xeta;
+    const double qi_zeta = R.c[4] * (pn.xsum - xm);
+    const double qj_xi = R.c[3] * pc.yxi;
+    const double qj_zeta = R.c[5] * (pn.ysum - ym);
+    const double qk_eta = R.c[5] * ((zc.n + zm.n) - (zc.s + zm.s));
+    const double qk_xi = R.c[4] * ((zc.e + zm.e) - (zc.w + zm.w));
+    const double m1 = R.c[6] * (0.5 * (pc.xc + pc.xw));
+    const double m2 = R.c[7] * (0.5 * (pc.yc + pc.ys));
+    const double m3 = R.c[8] * (0.5 * (zc.c + zm.c));
+    const double D = a1 + a2 + a3 + qi_eta + qi_zeta + qj_xi + qj_zeta + qk_eta + qk_xi + m1 + m2 + m3;
+    const double um = R.uprev;
+    double du = un - um;
+    bool approx = un == um;
+    if (!approx && isfinite(un) && isfinite(um))
+      approx = fabs(un - um) <= std::numeric_limits<double>::epsilon() * fmax(fabs(un), fabs(um));
+    if (approx) du = copysign(0.0, du);
+    double r = -du / dt - D + R.src;
+    const int64_t c = at(g, ci, cj, k);
+    if (A.tzA) r = r + (A.tzA[c] + A.tzB[c] * an);
+    A.r32[c] = (float)r;
+    sq += r * r;
+    const double ar = fabs(r);
+    mx = (r != r) ? r : nanmax(mx, ar);
+  };
+
+  // ---- prologue: plane step z0-1 (no zeta faces), then z0
+  CellIn C[2];
+  Plane pc{}, pn{};
+  ZSt zm{}, zc{};
+  double xm, ym;                // face sums of the plane below the next residual's
+  double u_c = 0.0, a_c = 0.0;  // column cell of the plane whose residual is next
+  fetch_cells(z0 - 1, C);
+  cells(z0 - 1, C);
+  fetch_cells(z0, C);
+  __syncthreads();
+  faces(z0 - 1, false);
+  __syncthreads();
+  read_plane(pn);
+  xm = pn.xsum;
+  ym = pn.ysum;
+  cells(z0, C);
+  u_c = u_own;
+  a_c = a_own;
+  fetch_cells(z0 + 1, C);
+  __syncthreads();
+  faces(z0, true);
+  __syncthreads();
+  read_plane(pc);
+  read_z(zm);
+  // ---- plane steps z0+1 .. z1+1: residual of plane p-1
+  for (int p = z0 + 1; p <= z1 + 1; ++p) {
+    ResIn R{};
+    fetch_res(p - 1, R);  // in flight during the cells and faces of plane p
+    cells(p, C);
+    if (p <= z1) fetch_cells(p + 1, C);
+    __syncthreads();
+    faces(p, true);
+    __syncthreads();
+    read_plane(pn);
+    read_z(zc);
+    residual(p - 1, R, xm, ym, pc, pn, zm, zc, u_c, a_c);
+    xm = pc.xsum;
+    ym = pc.ysum;
+    pc = pn;
+    zm = zc;
+    u_c = u_own;
+    a_c = a_own;
+  }
+
+  sq = wave_sum(sq);
+  mx = wave_max(mx);
+  if ((tid & (WAVE - 1)) == 0) {
+    sRed[0][tid / WAVE] = sq;
+    sRed[1][tid / WAVE] = mx;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0, m = 0.0;
+    for (int w = 0; w < L3T / WAVE; ++w) {
+      s += sRed[0][w];
+      m = nanmax(m, sRed[1][w]);
+    }
+    A.psum[b] = s;
+    A.pmax[b] = m;
+  }
+}
+
+}  // namespace cdiff

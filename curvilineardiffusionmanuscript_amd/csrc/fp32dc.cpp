@@ -354,6 +354,51 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
     LAUNCH_CHECK("k_alpha_full");
   }
   const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
+  if (ndim == 3) {
+    const int nk = g.hi[2] - g.lo[2] + 1;
+    DcOuter3D B{};
+    B.g = g;
+    B.gh = gh;
+    B.kap = kap;
+    B.ic = IterConst{L, dc_Vpdtau, dt};
+    B.harmonic = mean == CDIFF_MEAN_HARMONIC;
+    B.newton = dc_newton;
+    B.ch = std::max(1, std::min(nk, env_int("CDIFF_OUT3_CH", 32)));
+    B.stripsI = (ni + 63) / 64;
+    B.chunks = (nk + B.ch - 1) / B.ch;
+    B.ntiles = B.stripsI * ((nj + 7) / 8) * B.chunks;
+    B.t_exp = kap.t_exp;
+    B.rho = rho;
+    B.u_old = u_old;
+    B.e = e;
+    B.u_new = u_new;
+    B.a64 = a64;
+    B.uprev = uprev;
+    for (int c = 0; c < 9; ++c) B.c[c] = coef[c];
+    B.S = S;
+    B.tzA = tzA;
+    B.tzB = tzB;
+    B.r32 = base32;
+    for (int c = 0; c < 3; ++c) {
+      B.f[0][c] = lin_fx[c];
+      B.f[1][c] = lin_fy[c];
+      B.f[2][c] = lin_fz[c];
+    }
+    B.cc = lin_cc;
+    B.psum = dc_psum;
+    B.pmax = dc_pmax;
+    const int nblk = (B.ntiles + 7) / 8 * 8;
+    if (nblk > dc_nparts) return fail(CDIFF_HIP_ERROR, "partial-sum buffer too small");
+    hipLaunchKernelGGL(k_dc_outer3d, dim3(nblk), dim3(512), 0, stream, B);
+    LAUNCH_CHECK("k_dc_outer3d");
+    if (finalize) {
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl, (const double*)dc_psum,
+                         (const double*)dc_pmax, nblk, iter, INT_MAX, o.rtol, o.atol, N_full, o.norm_kind, 1);
+      LAUNCH_CHECK("k_finalize");
+    }
+    lin_coefs_ready = true;
+    return CDIFF_OK;
+  }
   DcOuter2D A{};
   A.g = g;
   A.gh = gh;
@@ -398,7 +443,8 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
 // residual of the iterate in u[dstbuf] after u[dstbuf] = u[srcbuf] + e
 int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o) {
   const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  if (lin && ndim == 2) return dc_outer_pass(dt, u[srcbuf], e, u[dstbuf], u_prev, iter, o, true);
+  if (lin && (ndim == 2 || env_int("CDIFF_DC_FUSED3", 1) != 0))
+    return dc_outer_pass(dt, u[srcbuf], e, u[dstbuf], u_prev, iter, o, true);
   hipLaunchKernelGGL(k_dc_update, grid_n(nfull), dim3(256), 0, stream, g, gh, kap, (const double*)rho,
                      (const double*)u[srcbuf], e, u[dstbuf], a64, a32, (dc_newton && !lin) ? ub32 : nullptr,
                      (dc_newton && !lin) ? da32 : nullptr);

@@ -23,52 +23,10 @@
 // between s and s+1) exist.  Every face is computed once per block; the ring
 // of the plane is recomputed by the neighbouring blocks.
 #include "cdiff_dev.h"
+#include "column3d.h"
 #include "sweep_args.h"
 
 namespace cdiff {
-
-namespace {
-
-constexpr int L3X = 64, L3Y = 8;
-constexpr int R3X = L3X + 2, R3Y = L3Y + 2, R3 = R3X * R3Y;  // ringed plane
-constexpr int L3T = L3X * L3Y;
-constexpr int RING = R3 - L3T;  // 148
-
-// region cell of a thread: slot 0 its owned cell, slot 1 a ring cell (tid < RING)
-__device__ __forceinline__ bool region_cell(int slot, int tid, int& lx, int& ly) {
-  if (slot == 0) {
-    lx = tid % L3X + 1;
-    ly = tid / L3X + 1;
-    return true;
-  }
-  int k = tid;
-  if (k < R3X) {
-    lx = k;
-    ly = 0;
-    return true;
-  }
-  k -= R3X;
-  if (k < R3X) {
-    lx = k;
-    ly = R3Y - 1;
-    return true;
-  }
-  k -= R3X;
-  if (k < L3Y) {
-    lx = 0;
-    ly = 1 + k;
-    return true;
-  }
-  k -= L3Y;
-  if (k < L3Y) {
-    lx = R3X - 1;
-    ly = 1 + k;
-    return true;
-  }
-  return false;
-}
-
-}  // namespace
 
 template <bool CHECK>
 __global__ __launch_bounds__(L3T) void sweep3d_lin_kernel(Lin3DArgs A) {
@@ -79,20 +37,16 @@ __global__ __launch_bounds__(L3T) void sweep3d_lin_kernel(Lin3DArgs A) {
   __shared__ double sRed[2][L3T / WAVE];
   const Geo& g = A.g;
   const int tid = threadIdx.x;
-  const int nb = gridDim.x, per = nb >> 3, b = blockIdx.x;
-  const int tile = (b & 7) * per + (b >> 3);
-  if (tile >= A.ntiles) {
+  const int b = blockIdx.x;
+  Column3D col3;
+  if (!column_of(g, A.ntiles, A.chunks, A.stripsI, A.ch, col3)) {
     if (CHECK && tid == 0) {
       A.part_sum[b] = 0.0;
       A.part_max[b] = 0.0;
     }
     return;
   }
-  const int kc = tile % A.chunks;
-  const int rest = tile / A.chunks;
-  const int si = rest % A.stripsI, sj = rest / A.stripsI;
-  const int x0 = g.lo[0] + si * L3X, y0 = g.lo[1] + sj * L3Y;
-  const int z0 = g.lo[2] + kc * A.ch, z1 = min(z0 + A.ch - 1, g.hi[2]);
+  const int x0 = col3.x0, y0 = col3.y0, z0 = col3.z0, z1 = col3.z1;
   const int gx = x0 - 1, gy = y0 - 1;
   const float inv_dt = A.inv_dt;
 
