@@ -224,9 +224,17 @@ int cdiff_create(int ndim, const int64_t* dims_full, int nhalo, int mean, const 
   if ((rc = alloc_field(h, &h->u_prev))) return rc;
   if ((rc = alloc_field(h, &h->Tdev))) return rc;
   if ((rc = alloc_field(h, &h->alpha))) return rc;
+  // the divergence coefficients in one block, array c at coef[0] + c * total
   const int ncoef = ndim == 2 ? 5 : 9;
-  for (int c = 0; c < ncoef; ++c)
-    if ((rc = alloc_field(h, &h->coef[c]))) return rc;
+  {
+    double* blk = nullptr;
+    HIP_TRY(hipMalloc((void**)&blk, sizeof(double) * h->g.total * ncoef));
+    h->owned.push_back(blk);
+    hipLaunchKernelGGL(k_fill, dim3((h->g.total * ncoef + 255) / 256), dim3(256), 0, h->stream, h->g.total * ncoef, 0.0,
+                       blk);
+    LAUNCH_CHECK("k_fill");
+    for (int c = 0; c < ncoef; ++c) h->coef[c] = blk + h->g.total * c;
+  }
   HIP_TRY(hipMalloc((void**)&h->ctrl, sizeof(Ctrl)));
   h->owned.push_back(h->ctrl);
   HIP_TRY(hipHostMalloc((void**)&h->ctrl_h, 4 * sizeof(Ctrl), hipHostMallocDefault));

@@ -83,8 +83,14 @@ int cdiff_solver::ensure_fp32() {
     if ((rc = alloc32(&da32))) return rc;
   }
   const int ncoef = ndim == 2 ? 5 : 9;
-  for (int c = 0; c < ncoef; ++c)
-    if ((rc = alloc32(&coef32[c]))) return rc;
+  if (!coef32[0]) {
+    // one block, array c at coef32[0] + c * total (as the FP64 coefficients)
+    float* blk = nullptr;
+    HIP_TRY(hipMalloc((void**)&blk, sizeof(float) * g.total * ncoef));
+    owned32.push_back(blk);
+    for (int c = 0; c < ncoef; ++c) coef32[c] = blk + g.total * c;
+    coefs32_valid = false;
+  }
   if (!a64) {
     HIP_TRY(hipMalloc((void**)&a64, sizeof(double) * g.total));
     owned.push_back(a64);
@@ -254,7 +260,11 @@ int cdiff_solver::launch_sweep32(double dt, double Vpdtau, int src, bool check, 
   A.part_sum = part_sum;
   A.part_max = part_max;
   A.ctrl = ctrl_in;
-  launch_sweep3d<float>(A, check, nblocks, stream);
+  A.coef_stride = g.total;
+  if (col3d)
+    launch_sweep3d_col<float>(A, check, nblocks, stream);
+  else
+    launch_sweep3d<float>(A, check, nblocks, stream);
   LAUNCH_CHECK("sweep3d<float>");
   return CDIFF_OK;
 }

@@ -84,13 +84,28 @@ void cdiff_solver::configure_sweep() {
     nblocks = (ntiles + 7) / 8 * 8;
   } else {
     const int nk = g.hi[2] - g.lo[2] + 1;
-    strips3[0] = (ni + OWN - 1) / OWN;
-    strips3[1] = (nj + (SWEEP3D_BJ - 2) - 1) / (SWEEP3D_BJ - 2);
+    {
+      const char* v = std::getenv("CDIFF_SWEEP3D");
+      col3d = !(v && std::string(v) == "lanes");
+    }
     int c = env_int("CDIFF_CH3", 0);
-    if (c <= 0) {
-      // aim for ~2048 blocks of SWEEP3D_BJ waves, 4..64 planes per chunk
-      const int64_t cols = (int64_t)strips3[0] * strips3[1];
-      c = (int)std::min<int64_t>(64, std::max<int64_t>(4, (cols * nk + 2047) / 2048));
+    if (col3d) {
+      strips3[0] = (ni + 63) / 64;
+      strips3[1] = (nj + 7) / 8;
+      if (c <= 0) {
+        // aim for ~1024 blocks of 512 threads (one LDS-heavy block per CU at a
+        // time), 4..64 planes per chunk
+        const int64_t cols = (int64_t)strips3[0] * strips3[1];
+        c = (int)std::min<int64_t>(64, std::max<int64_t>(4, (cols * nk + 1023) / 1024));
+      }
+    } else {
+      strips3[0] = (ni + OWN - 1) / OWN;
+      strips3[1] = (nj + (SWEEP3D_BJ - 2) - 1) / (SWEEP3D_BJ - 2);
+      if (c <= 0) {
+        // aim for ~2048 blocks of SWEEP3D_BJ waves, 4..64 planes per chunk
+        const int64_t cols = (int64_t)strips3[0] * strips3[1];
+        c = (int)std::min<int64_t>(64, std::max<int64_t>(4, (cols * nk + 2047) / 2048));
+      }
     }
     c = std::min(c, nk);
     ch = c;
@@ -191,7 +206,11 @@ int cdiff_solver::launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, 
   A.part_sum = part_sum;
   A.part_max = part_max;
   A.ctrl = ctrl;
-  launch_sweep3d<double>(A, check, nblocks, stream);
+  A.coef_stride = g.total;
+  if (col3d)
+    launch_sweep3d_col<double>(A, check, nblocks, stream);
+  else
+    launch_sweep3d<double>(A, check, nblocks, stream);
   LAUNCH_CHECK("sweep3d");
   return CDIFF_OK;
 }

@@ -71,6 +71,7 @@ struct Sweep3DArgs {
   T* __restrict__ qz_dst;
   const T* __restrict__ u_prev;
   const T* coef[9];  // g11 g22 g33, 0.25*g_xi.eta, 0.25*g_xi.zeta, 0.25*g_eta.zeta, alpha_c beta_c gamma_c
+  int64_t coef_stride;  // coef[c] == coef[0] + c * coef_stride (one block)
   const T* __restrict__ S;
   const T* __restrict__ rho;
   const T* __restrict__ tzA;
@@ -90,6 +91,9 @@ struct Sweep3DArgs {
 
 template <typename T>
 void launch_sweep3d(const Sweep3DArgs<T>& A, bool check, int nblocks, hipStream_t s);
+// the same sweep over 64 x 8 columns with one-cell rings (sweep3d_col.hip)
+template <typename T>
+void launch_sweep3d_col(const Sweep3DArgs<T>& A, bool check, int nblocks, hipStream_t s);
 
 template <typename T>
 void launch_sweep2d_pair(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_t s);

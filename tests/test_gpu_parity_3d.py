@@ -15,6 +15,14 @@ pytestmark = pytest.mark.gpu
 _DEV = {"neumann": S.NeumannBC(), "periodic": S.PeriodicBC()}
 
 
+@pytest.fixture(params=["col", "lanes"], autouse=True)
+def sweep3d_kernel(request, monkeypatch):
+    # col: 64 x 8 columns with one-cell rings (default); lanes: one lane per
+    # column with lane/wave rings.  Both must reproduce the oracle bit for bit.
+    monkeypatch.setenv("CDIFF_SWEEP3D", request.param)
+    return request.param
+
+
 def _pair(mesh, T0, spec, kappa, nsteps, dt, **opts):
     bd = [S.DirichletBC(s[1]) if isinstance(s, tuple) else _DEV[s] for s in spec]
     bo = [("dirichlet", s[1]) if isinstance(s, tuple) else (s, 0.0) for s in spec]
