@@ -67,10 +67,12 @@ struct DcOuter3D {
   double* u_new;
   double* a64;
   const double* uprev;
-  const double* c[9];  // g11 g22 g33 g12 g13 g23 ac bc gc
+  const double* c;  // g11 g22 g33 g12 g13 g23 ac bc gc: array q at c + q * cstride
+  int64_t cstride;
   const double *S, *tzA, *tzB;
   float* r32;
-  float* f[3][3];  // P, M, W per axis
+  float* f;  // [axis][P, M, W]: array (axis, c) at f + (3 axis + c) * fstride
+  int64_t fstride;
   float* cc;
   double* psum;
   double* pmax;

@@ -224,7 +224,7 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
           const int r = cy * R3X + fx;
           v = -face_mean<double>(harm, sA[bc][r], sA[bc][r + 1]) * (sU[bc][r + 1] - sU[bc][r]);
           if (own_p && cy >= 1 && cy <= L3Y && (fx >= 1 || i == g.lo[0] - 1))
-            coefs(bc, r, bc, r + 1, 0, 1, i == g.lo[0] - 1, i == g.hi[0], A.f[0][0], A.f[0][1], A.f[0][2], at(g, i, j, p));
+            coefs(bc, r, bc, r + 1, 0, 1, i == g.lo[0] - 1, i == g.hi[0], A.f + 0 * A.fstride, A.f + 1 * A.fstride, A.f + 2 * A.fstride, at(g, i, j, p));
         }
         sX[t] = v;
       } else if (t < NFX + NFY) {
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
           const int r = fy * R3X + fx;
           v = -face_mean<double>(harm, sA[bc][r], sA[bc][r + R3X]) * (sU[bc][r + R3X] - sU[bc][r]);
           if (own_p && fx >= 1 && fx <= L3X && (fy >= 1 || j == g.lo[1] - 1))
-            coefs(bc, r, bc, r + R3X, 2, 3, j == g.lo[1] - 1, j == g.hi[1], A.f[1][0], A.f[1][1], A.f[1][2], at(g, i, j, p));
+            coefs(bc, r, bc, r + R3X, 2, 3, j == g.lo[1] - 1, j == g.hi[1], A.f + 3 * A.fstride, A.f + 4 * A.fstride, A.f + 5 * A.fstride, at(g, i, j, p));
         }
         sY[u] = v;
       } else if (t < NFACE) {
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
           v = -face_mean<double>(harm, sA[bm][r], sA[bc][r]) * (sU[bc][r] - sU[bm][r]);
           const bool own_k = (k >= z0 && k <= z1) || k == g.lo[2] - 1;
           if (own_k && lx >= 1 && lx <= L3X && ly >= 1 && ly <= L3Y)
-            coefs(bm, r, bc, r, 4, 5, k == g.lo[2] - 1, k == g.hi[2], A.f[2][0], A.f[2][1], A.f[2][2], at(g, i, j, k));
+            coefs(bm, r, bc, r, 4, 5, k == g.lo[2] - 1, k == g.hi[2], A.f + 6 * A.fstride, A.f + 7 * A.fstride, A.f + 8 * A.fstride, at(g, i, j, k));
         }
         sZ[r] = v;
       }
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
     if (!col || p < z0 || p > z1) return;
     const int64_t c = at(g, ci, cj, p);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) R.c[q] = A.c[q][c];
+    for (int q = 0; q < 9; ++q) R.c[q] = A.c[q * A.cstride + c];
     R.uprev = A.uprev[c];
     R.src = A.S ? A.S[c] : 0.0;
   };

@@ -141,13 +141,30 @@ int cdiff_solver::ensure_fp32() {
   }
   if (lin) {
     lin_nblocks = (lin_tiles + 7) / 8 * 8;
+    // one block each: the folded metrics m[c] at m[0] + c * total; the face
+    // coefficients [axis][P, M, W] at fx[0] + (3 axis + c) * total
+    auto alloc_block = [&](float** p, int n) -> int {
+      HIP_TRY(hipMalloc((void**)p, sizeof(float) * g.total * n));
+      owned32.push_back(*p);
+      hipLaunchKernelGGL(k_fill32, grid_n(g.total * n), dim3(256), 0, stream, g.total * n, 0.0f, *p);
+      LAUNCH_CHECK("k_fill32");
+      return CDIFF_OK;
+    };
     const int nm = ndim == 2 ? 5 : 9;
-    for (int c = 0; c < nm; ++c)
-      if ((rc = alloc32(&lin_m[c]))) return rc;
-    for (int c = 0; c < 3; ++c) {
-      if ((rc = alloc32(&lin_fx[c]))) return rc;
-      if ((rc = alloc32(&lin_fy[c]))) return rc;
-      if (ndim == 3 && (rc = alloc32(&lin_fz[c]))) return rc;
+    if (!lin_m[0]) {
+      float* blk = nullptr;
+      if ((rc = alloc_block(&blk, nm))) return rc;
+      for (int c = 0; c < nm; ++c) lin_m[c] = blk + g.total * c;
+      lin_metrics_valid = false;
+    }
+    if (!lin_fx[0]) {
+      float* blk = nullptr;
+      if ((rc = alloc_block(&blk, 3 * ndim))) return rc;
+      for (int c = 0; c < 3; ++c) {
+        lin_fx[c] = blk + g.total * c;
+        lin_fy[c] = blk + g.total * (3 + c);
+        if (ndim == 3) lin_fz[c] = blk + g.total * (6 + c);
+      }
     }
     if ((rc = alloc32(&lin_cc))) return rc;
     if (lin_nblocks > lin_nparts) {
@@ -307,12 +324,9 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit) {
     }
     B.base = base32;
     B.cc = lin_cc;
-    for (int c = 0; c < 9; ++c) B.m[c] = lin_m[c];
-    for (int c = 0; c < 3; ++c) {
-      B.f[0][c] = lin_fx[c];
-      B.f[1][c] = lin_fy[c];
-      B.f[2][c] = lin_fz[c];
-    }
+    B.m = lin_m[0];
+    B.f = lin_fx[0];
+    B.stride = g.total;
     B.part_sum = lin_psum;
     B.part_max = lin_pmax;
     B.ctrl = ctrl_in;
@@ -384,16 +398,14 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
     B.u_new = u_new;
     B.a64 = a64;
     B.uprev = uprev;
-    for (int c = 0; c < 9; ++c) B.c[c] = coef[c];
+    B.c = coef[0];
+    B.cstride = g.total;
     B.S = S;
     B.tzA = tzA;
     B.tzB = tzB;
     B.r32 = base32;
-    for (int c = 0; c < 3; ++c) {
-      B.f[0][c] = lin_fx[c];
-      B.f[1][c] = lin_fy[c];
-      B.f[2][c] = lin_fz[c];
-    }
+    B.f = lin_fx[0];
+    B.fstride = g.total;
     B.cc = lin_cc;
     B.psum = dc_psum;
     B.pmax = dc_pmax;
@@ -476,11 +488,14 @@ int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf,
 int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cdiff_stats* st) {
   int rc = ensure_fp32();
   if (rc) return rc;
-  const double eta = env_double("CDIFF_DC_ETA", 1e-3);
+  // inner stop at eta times the outer residual, checked every ichk sweeps; in
+  // 3D (one sweep per launch) shorter inner solves checked more often need
+  // fewer sweeps in total (256^3 spots: 48 -> 32 sweeps per step)
+  const double eta = env_double("CDIFF_DC_ETA", ndim == 3 ? 1e-2 : 1e-3);
   // the last correction only needs to reach the outer tolerance: inner
   // sweeps stop at floor_frac * atol at the latest
   const double floor_frac = env_double("CDIFF_DC_FLOOR", 0.3);
-  const int ichk = std::max(1, env_int("CDIFF_DC_CHECK", 8));
+  const int ichk = std::max(1, env_int("CDIFF_DC_CHECK", ndim == 3 ? 2 : 8));
   const int max_outer = std::max(1, env_int("CDIFF_DC_MAX_OUTER", 100));
   const int fixed_inner = o.inner_iters;
   const int64_t total_cap = (int64_t)o.max_iter;
@@ -488,7 +503,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   const int lin_nit = std::max(1, std::min(lin_k, env_int("CDIFF_LIN_NIT", lin_k)));
   // pseudo-time step of the correction sweeps: the fixed point does not depend
   // on it, so it may differ from the FP64 APT's (scale 1 = the reference's)
-  if (dc_cfl_scale < 0) dc_cfl_scale = env_double("CDIFF_DC_CFL_SCALE", ndim == 2 ? 1.2 : 1.0);
+  if (dc_cfl_scale < 0) dc_cfl_scale = env_double("CDIFF_DC_CFL_SCALE", 1.2);
   dc_Vpdtau = Vpdtau * dc_cfl_scale;
   // u[0] holds T (prologue); the first FP64 residual, of T with its BCs, into u[1]
   int a = 1;

@@ -80,23 +80,23 @@ __global__ __launch_bounds__(L3T) void sweep3d_lin_kernel(Lin3DArgs A) {
       if (in_dom(i, j, k)) F[p].e = A.e_src[at(g, i, j, k)];
       if (rlx[p] < R3X - 1 && face_on(0, i, j, k)) {
         const int64_t c = at(g, i, j, k);
-        F[p].f[0][0] = A.f[0][0][c];
-        F[p].f[0][1] = A.f[0][1][c];
-        F[p].f[0][2] = A.f[0][2][c];
+        F[p].f[0][0] = A.f[0 * A.stride + c];
+        F[p].f[0][1] = A.f[1 * A.stride + c];
+        F[p].f[0][2] = A.f[2 * A.stride + c];
         F[p].f[0][3] = A.q_src[0][c];
       }
       if (rly[p] < R3Y - 1 && face_on(1, i, j, k)) {
         const int64_t c = at(g, i, j, k);
-        F[p].f[1][0] = A.f[1][0][c];
-        F[p].f[1][1] = A.f[1][1][c];
-        F[p].f[1][2] = A.f[1][2][c];
+        F[p].f[1][0] = A.f[3 * A.stride + c];
+        F[p].f[1][1] = A.f[4 * A.stride + c];
+        F[p].f[1][2] = A.f[5 * A.stride + c];
         F[p].f[1][3] = A.q_src[1][c];
       }
       if (zf && face_on(2, i, j, k - 1)) {
         const int64_t c = at(g, i, j, k - 1);
-        F[p].f[2][0] = A.f[2][0][c];
-        F[p].f[2][1] = A.f[2][1][c];
-        F[p].f[2][2] = A.f[2][2][c];
+        F[p].f[2][0] = A.f[6 * A.stride + c];
+        F[p].f[2][1] = A.f[7 * A.stride + c];
+        F[p].f[2][2] = A.f[8 * A.stride + c];
         F[p].f[2][3] = A.q_src[2][c];
       }
     }
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(L3T) void sweep3d_lin_kernel(Lin3DArgs A) {
     U.rb = A.base[c];
     U.cc = A.cc[c];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) U.m[q] = A.m[q][c];
+    for (int q = 0; q < 9; ++q) U.m[q] = A.m[q * A.stride + c];
   };
 
   // ---- prologue: plane steps z0-1, z0, z0+1 (the zeta faces below z0-1 are

@@ -142,8 +142,9 @@ struct Lin3DArgs {
   float* q_dst[3];
   const float* __restrict__ base;
   const float* __restrict__ cc;
-  const float* m[9];     // m0..m5 (x_c, x_w, y_c, y_s, z_c, z_s), cxe, cxz, cez
-  const float* f[3][3];  // [axis][P, M, W]
+  const float* m;  // m0..m5 (x_c, x_w, y_c, y_s, z_c, z_s), cxe, cxz, cez: array q at m + q * stride
+  const float* f;  // [axis][P, M, W]: array (axis, c) at f + (3 axis + c) * stride
+  int64_t stride;
   double* part_sum;
   double* part_max;
   const Ctrl* ctrl;
