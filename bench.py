@@ -8,6 +8,9 @@ T = 1 on the left edge, zero-flux elsewhere, adaptive dt from next_dt
 (dt0 = 1e-6), residual tolerance 1e-8 (atol on the FP64 RMS residual,
 rtol = 0), 10 steps.  A "step" is one time step: the full pseudo-transient
 solve to tolerance.  The temperature stays in HBM for the whole timed region.
+`--workload spots_3d` runs the 3D mixed-precision case instead: 256^3 cells,
+x = xi + 0.04 sin(2 pi xi) sin(2 pi eta) sin(2 pi zeta) (same for y, z),
+kappa = T^3, 8 seeded Gaussian spots on T = 0.01, zero-flux faces, dt0 = 1e-5.
 
 Timed in the same run: the headline method (FP32 correction sweeps inside
 the FP64 defect-correction loop) and the plain FP64 APT, from the same
@@ -40,10 +43,29 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {("fp64", 2): 96, ("fp32_dc", 2): 48, ("fp64", 3): 144, ("fp32_dc", 3): 72}
 
 
-def make_workload(n, seed):
+# workloads: mesh and T0 builder, boundary conditions (device, oracle), kappa
+# law (kappa0, T exponent, rho exponent), dimension, default size and dt0
+WORKLOADS = {
+    "hot_disk_2d": dict(ndim=2, n=4096, dt0=1e-6, kappa=(1.0, 2.5, 0.0),
+                        bcs=[("dirichlet", 1.0)] + [("neumann", 0.0)] * 3),
+    "spots_3d": dict(ndim=3, n=256, dt0=1e-5, kappa=(1.0, 3.0, 0.0), bcs=[("neumann", 0.0)] * 6),
+}
+
+
+def make_workload(name, n, seed):
+    if name == "spots_3d":
+        from cases import gaussian_spots_3d
+
+        return gaussian_spots_3d(n, seed=seed)
     from cases import hot_disk_2d
 
     return hot_disk_2d(n, seed=seed)
+
+
+def device_bcs(spec):
+    from curvilineardiffusionmanuscript_amd import solver as S
+
+    return [S.DirichletBC(v) if kind == "dirichlet" else S.NeumannBC() for kind, v in spec]
 
 
 def log(*a):
@@ -53,9 +75,9 @@ def log(*a):
 def run_gpu(args, mesh, T0, method):
     from curvilineardiffusionmanuscript_amd import solver as S
 
-    law = S.PowerLawConductivity(1.0, 2.5, 0.0)
-    bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
-    sol = S.PseudoTransientSolver(mesh, bcs)
+    w = WORKLOADS[args.workload]
+    law = S.PowerLawConductivity(*w["kappa"])
+    sol = S.PseudoTransientSolver(mesh, device_bcs(w["bcs"]))
     kw = dict(rtol=0.0, atol=args.tol, max_iter=10**7, precision=method)
 
     def reset():
@@ -90,10 +112,13 @@ def run_gpu(args, mesh, T0, method):
                 abs_err=[s[2] for s in steps])
 
 
-# dominant kernel of each method, with the pseudo-iterations one launch runs
+# dominant kernel of each method and dimension, with the pseudo-iterations
+# one launch runs and the cdiff_bench_sweeps mode bits that launch it
 KERNELS = {
-    "fp64": ("sweep2d_pair_kernel<double, false, 8>", 2, 0 | 2),
-    "fp32_dc": ("sweep2d_lin_kernel<4, 16, false>", 4, 4),
+    ("fp64", 2): ("sweep2d_pair_kernel<double, false, 8>", 2, 0 | 2),
+    ("fp32_dc", 2): ("sweep2d_lin_kernel<4, 16, false>", 4, 4),
+    ("fp64", 3): ("sweep3d_col_kernel<double, false, true>", 1, 0),
+    ("fp32_dc", 3): ("sweep3d_lin_kernel<false>", 1, 4),
 }
 
 
@@ -105,7 +130,7 @@ def sol_bench(sol, dt, method, n):
     out = C.c_double()
     prec = L.PREC_FP64 if method == "fp64" else L.PREC_FP32_DC
     n = n - n % 4
-    rc = L.load().cdiff_bench_sweeps(sol._h, float(dt), prec, KERNELS[method][2], int(n), C.byref(out))
+    rc = L.load().cdiff_bench_sweeps(sol._h, float(dt), prec, KERNELS[(method, sol.ndim)][2], int(n), C.byref(out))
     if rc:
         raise RuntimeError(L.load().cdiff_last_error(sol._h).decode())
     return out.value
@@ -118,7 +143,8 @@ def run_cpu(args, mesh, T0, budget_s):
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     O.set_num_threads(threads)
-    s = O.OracleSolver(mesh, [("dirichlet", 1.0)] + [("neumann", 0.0)] * 3, kappa=(1.0, 2.5, 0.0))
+    w = WORKLOADS[args.workload]
+    s = O.OracleSolver(mesh, w["bcs"], kappa=w["kappa"])
     T = T0.copy(order="F")
     s.update_conductivity(T)
     dt = args.dt0
@@ -134,13 +160,13 @@ def run_cpu(args, mesh, T0, budget_s):
     return dict(s_per_step=el / nsteps, steps=nsteps, iters=nit, threads=O.num_threads(), elapsed=el)
 
 
-def load_traffic(kernel, n):
-    """Measured HBM bytes per launch of `kernel` at n x n (rocprofv3 counters,
+def load_traffic(kernel, n, ndim):
+    """Measured HBM bytes per launch of `kernel` at n^ndim (rocprofv3 counters,
     calibrated against a copy kernel), if profiles/traffic.json has them."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get(f"{kernel} @ {n}x{n}")
+            return json.load(f).get(f"{kernel} @ {'x'.join([str(n)] * ndim)}")
     return None
 
 
@@ -149,9 +175,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=4096, help="cells per axis")
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--dt0", type=float, default=1e-6)
+    ap.add_argument("--workload", default="hot_disk_2d", choices=sorted(WORKLOADS),
+                    help="hot_disk_2d: the headline 2D case; spots_3d: the 3D mixed-precision case")
+    ap.add_argument("--n", type=int, default=None, help="cells per axis (default: 4096 in 2D, 256 in 3D)")
+    ap.add_argument("--seed", type=int, default=None, help="workload seed (default: 0 in 2D, 1 in 3D)")
+    ap.add_argument("--dt0", type=float, default=None, help="first time step (default: 1e-6 in 2D, 1e-5 in 3D)")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--method", default="fp32_dc", choices=["fp64", "fp32_dc"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 APT comparison run")
@@ -167,15 +195,23 @@ def main():
         )
         sys.exit(2)
 
+    w = WORKLOADS[args.workload]
+    ndim = w["ndim"]
+    if args.n is None:
+        args.n = w["n"]
+    if args.seed is None:
+        args.seed = 0 if ndim == 2 else 1
+    if args.dt0 is None:
+        args.dt0 = w["dt0"]
     t_setup = time.perf_counter()
-    mesh, T0 = make_workload(args.n, args.seed)
-    log(f"workload {args.n}^2 built in {time.perf_counter() - t_setup:.1f}s")
+    mesh, T0 = make_workload(args.workload, args.n, args.seed)
+    log(f"workload {args.workload} {args.n}^{ndim} built in {time.perf_counter() - t_setup:.1f}s")
     g = run_gpu(args, mesh, T0, args.method)
     ref = None
     if args.method != "fp64" and not args.no_fp64:
         ref = run_gpu(args, mesh, T0, "fp64")
-    n_cells = args.n * args.n
-    bpc = BYTES_PER_CELL[(args.method, 2)]
+    n_cells = args.n**ndim
+    bpc = BYTES_PER_CELL[(args.method, ndim)]
     s_per_step = g["elapsed"] / args.steps
     it_mean = float(np.mean(g["iters"]))
     teff = bpc * n_cells * sum(g["iters"]) / g["elapsed"] / 1e9
@@ -195,12 +231,12 @@ def main():
             "max_rel_diff_vs_fp64": err,
         }
 
-    kname, per_launch, _ = KERNELS[args.method]
+    kname, per_launch, _ = KERNELS[(args.method, ndim)]
     per_it_ms = g["sweep_ms"]
     launch_ms = per_it_ms * per_launch if per_it_ms == per_it_ms else None
     alg_bytes = bpc * n_cells * per_launch
     achieved = alg_bytes / (launch_ms / 1e3) / 1e9 if launch_ms else None
-    traffic = load_traffic(kname, args.n)
+    traffic = load_traffic(kname, args.n, ndim)
     roofline = {
         "kernel": kname,
         "bound": "hbm",
@@ -239,11 +275,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.method == "fp64" else "f32+f64",
-        "data": "synthetic (seeded hot disk, analytic wavy mesh)",
+        "data": "synthetic (seeded hot disk, analytic wavy mesh)" if ndim == 2
+        else "synthetic (seeded Gaussian spots, analytic wavy mesh)",
         "config": {
-            "workload": f"2D {args.n}x{args.n} wavy grid (A=0.05), kappa=T^2.5, hot disk seed {args.seed}, "
-            f"Dirichlet T=1 left, zero-flux elsewhere, adaptive dt0={args.dt0:g}, atol={args.tol:g}, "
-            f"method={args.method}",
+            "workload": (
+                f"2D {args.n}x{args.n} wavy grid (A=0.05), kappa=T^2.5, hot disk seed {args.seed}, "
+                f"Dirichlet T=1 left, zero-flux elsewhere, adaptive dt0={args.dt0:g}, atol={args.tol:g}, "
+                f"method={args.method}"
+                if ndim == 2
+                else f"3D {args.n}^3 wavy grid (A=0.04), kappa=T^3, 8 Gaussian spots seed {args.seed}, "
+                f"zero-flux, adaptive dt0={args.dt0:g}, atol={args.tol:g}, method={args.method}"
+            ),
             "iters_per_step": round(it_mean, 2),
             "outer_per_step": round(float(np.mean(g["outer"])), 2),
             "max_abs_err": max(g["abs_err"]),
