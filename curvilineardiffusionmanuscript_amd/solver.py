@@ -449,14 +449,14 @@ def nonlinear_thermal_conduction_step(
     nd = C.c_double(0.0)
     buf = T if T.flags.f_contiguous else np.asfortranarray(T)
     rc = L.load().cdiff_step(solver._h, L.ptr(buf), float(dt), C.byref(o), C.byref(st), C.byref(nd))
-    if kwargs.get("write_diagnostic_vtk") or rc in (L.NONFINITE, L.MAX_ITER):
-        try:
-            from .vtk import to_vtk
+    # the reference dumps the state before raising on a non-finite error or the
+    # iteration limit (PseudoTransientSolver.jl:302-314); with
+    # write_diagnostic_vtk it dumps at every error check (:298-300), which the
+    # device loop does not stop for: here the step's final state is dumped once
+    if rc in (L.NONFINITE, L.MAX_ITER) or (rc == L.OK and kwargs.get("write_diagnostic_vtk")):
+        from .vtk import to_vtk
 
-            if rc in (L.NONFINITE, L.MAX_ITER):
-                to_vtk(solver, mesh, solver.u, rho, st.niter, st.niter)
-        except Exception:
-            pass
+        to_vtk(solver, mesh, solver.u, 1.0 if rho is None else rho, st.niter, st.niter)
     _check(rc, solver._h)
     if buf is not T:
         T[...] = buf
