@@ -268,6 +268,44 @@ int cdiff_destroy(cdiff_solver* h) {
   return CDIFF_OK;
 }
 
+// divergence coefficients (and the 2D cell Jacobian) from metric arrays already
+// on the device; frees `tmp` (the metric arrays) when done
+static int finish_metrics(cdiff_solver* h, const MetricIn& m, std::vector<double*>& tmp, double L,
+                          const double* spacing) {
+  const int nd = h->ndim;
+  auto cleanup = [&]() {
+    for (double* p : tmp) (void)hipFree(p);
+    tmp.clear();
+  };
+  if (nd == 2) {
+    // cell Jacobian 1 / det(d(xi, eta)/d(x, y)), a feature of the learned warm start
+    if (!h->jac) {
+      int rc = alloc_field(h, &h->jac);
+      if (rc) {
+        cleanup();
+        return rc;
+      }
+    }
+    hipLaunchKernelGGL(k_cell_jacobian2d, grid_full(h->g), dim3(256), 0, h->stream, h->g, (const double*)m.cell[0],
+                       (const double*)m.cell[1], (const double*)m.cell[2], (const double*)m.cell[3], h->jac);
+  }
+  double* c[9] = {nullptr};
+  for (int t = 0; t < 9; ++t) c[t] = t < (nd == 2 ? 5 : 9) ? h->coef[t] : h->coef[0];
+  hipLaunchKernelGGL(k_metric_coefs, grid_full(h->g), dim3(256), 0, h->stream, h->g, m, c[0], c[1], c[2], c[3], c[4],
+                     c[5], c[6], c[7], c[8]);
+  hipError_t le = hipGetLastError();
+  hipError_t se = hipStreamSynchronize(h->stream);
+  cleanup();
+  if (le != hipSuccess) return h->hip_fail(le, "launch of k_metric_coefs");
+  if (se != hipSuccess) return h->hip_fail(se, "k_metric_coefs");
+  h->L = L;
+  for (int a = 0; a < nd; ++a) h->spacing[a] = spacing[a];
+  h->metrics_set = true;
+  h->coefs32_valid = false;
+  h->lin_metrics_valid = false;
+  return CDIFF_OK;
+}
+
 int cdiff_set_metrics(cdiff_solver* h, const double* const* cell, const double* const* edge, double L,
                       const double* spacing) {
   if (!h || !cell || !edge || !spacing) return CDIFF_BAD_ARG;
@@ -301,33 +339,65 @@ int cdiff_set_metrics(cdiff_solver* h, const double* const* cell, const double* 
     else
       m.edge[t - nm] = d;
   }
-  if (nd == 2) {
-    // cell Jacobian 1 / det(d(xi, eta)/d(x, y)), a feature of the learned warm start
-    if (!h->jac) {
-      int rc = alloc_field(h, &h->jac);
-      if (rc) {
-        cleanup();
-        return rc;
-      }
+  return finish_metrics(h, m, tmp, L, spacing);
+}
+
+int cdiff_set_nodes(cdiff_solver* h, const double* const* nodes, double L, const double* spacing) {
+  if (!h || !nodes || !spacing) return CDIFF_BAD_ARG;
+  const int nd = h->ndim, nm = nd * nd;
+  const int nedge = nd * (nm + 1);
+  for (int c = 0; c < nd; ++c)
+    if (!nodes[c]) return h->fail(CDIFF_BAD_ARG, "null node array");
+  const Geo& g = h->g;
+  const int64_t s1 = g.n[0] + 1, s2 = s1 * (g.n[1] + 1);
+  const int64_t nnodes = s2 * (nd == 3 ? g.n[2] + 1 : 1);
+  std::vector<double*> tmp;
+  auto cleanup = [&]() {
+    for (double* p : tmp) (void)hipFree(p);
+  };
+  auto dev = [&](size_t n, double** p) -> int {
+    hipError_t e = hipMalloc((void**)p, sizeof(double) * n);
+    if (e != hipSuccess) {
+      cleanup();
+      return h->hip_fail(e, "hipMalloc(nodes/metrics)");
     }
-    hipLaunchKernelGGL(k_cell_jacobian2d, grid_full(h->g), dim3(256), 0, h->stream, h->g, (const double*)m.cell[0],
-                       (const double*)m.cell[1], (const double*)m.cell[2], (const double*)m.cell[3], h->jac);
+    tmp.push_back(*p);
+    return CDIFF_OK;
+  };
+  double* X[3] = {nullptr, nullptr, nullptr};
+  int rc;
+  for (int c = 0; c < nd; ++c) {
+    if ((rc = dev((size_t)nnodes, &X[c]))) return rc;
+    hipError_t e = hipMemcpyAsync(X[c], nodes[c], sizeof(double) * nnodes, hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) {
+      cleanup();
+      return h->hip_fail(e, "hipMemcpyAsync(nodes)");
+    }
   }
-  double* c[9] = {nullptr};
-  for (int t = 0; t < 9; ++t) c[t] = t < (nd == 2 ? 5 : 9) ? h->coef[t] : h->coef[0];
-  hipLaunchKernelGGL(k_metric_coefs, grid_full(h->g), dim3(256), 0, h->stream, h->g, m, c[0], c[1], c[2], c[3], c[4],
-                     c[5], c[6], c[7], c[8]);
+  MetricIn m{};
+  MetricOut o{};
+  for (int t = 0; t < nm + nedge; ++t) {
+    double* d = nullptr;
+    if ((rc = dev((size_t)g.total, &d))) return rc;
+    if (t < nm) {
+      m.cell[t] = o.cell[t] = d;
+    } else {
+      m.edge[t - nm] = o.edge[t - nm] = d;
+    }
+  }
+  const int64_t ncell = (int64_t)g.n[0] * g.n[1] * (nd == 3 ? g.n[2] : 1);
+  if (nd == 2)
+    hipLaunchKernelGGL(k_node_metrics2d, dim3((ncell + 255) / 256), dim3(256), 0, h->stream, g, (const double*)X[0],
+                       (const double*)X[1], s1, o);
+  else
+    hipLaunchKernelGGL(k_node_metrics3d, dim3((ncell + 255) / 256), dim3(256), 0, h->stream, g, (const double*)X[0],
+                       (const double*)X[1], (const double*)X[2], s1, s2, o);
   hipError_t le = hipGetLastError();
-  hipError_t se = hipStreamSynchronize(h->stream);
-  cleanup();
-  if (le != hipSuccess) return h->hip_fail(le, "launch of k_metric_coefs");
-  if (se != hipSuccess) return h->hip_fail(se, "k_metric_coefs");
-  h->L = L;
-  for (int a = 0; a < nd; ++a) h->spacing[a] = spacing[a];
-  h->metrics_set = true;
-  h->coefs32_valid = false;
-  h->lin_metrics_valid = false;
-  return CDIFF_OK;
+  if (le != hipSuccess) {
+    cleanup();
+    return h->hip_fail(le, "launch of k_node_metrics");
+  }
+  return finish_metrics(h, m, tmp, L, spacing);
 }
 
 static int set_optional(cdiff_solver* h, double** slot, const double* src) {

@@ -9,6 +9,15 @@ struct MetricIn {
   const double* edge[30];
 };
 
+struct MetricOut {
+  double* cell[9];
+  double* edge[30];
+};
+// cell and face metrics from the ghost-inclusive node grid (metrics.hip)
+__global__ void k_node_metrics2d(Geo g, const double* X, const double* Y, int64_t s1, MetricOut out);
+__global__ void k_node_metrics3d(Geo g, const double* X, const double* Y, const double* Z, int64_t s1, int64_t s2,
+                                 MetricOut out);
+
 struct Coefs {
   const double* c[9];
 };

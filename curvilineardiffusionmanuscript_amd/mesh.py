@@ -94,11 +94,32 @@ class CurvilinearGrid2D:
                 full=(slice(0, NI), slice(0, NJ)), domain=(slice(h, h + ni), slice(h, h + nj))
             )
         )
-        self._build(X, Y)
+        self._XG, self._YG = X, Y
+        xc = 0.25 * (X[:-1, :-1] + X[1:, :-1] + X[:-1, 1:] + X[1:, 1:])
+        yc = 0.25 * (Y[:-1, :-1] + Y[1:, :-1] + Y[:-1, 1:] + Y[1:, 1:])
+        self.centroid_coordinates = SimpleNamespace(x=_f(xc), y=_f(yc))
+        self._metrics = None
 
-    # -- metrics -----------------------------------------------------------
-    def _build(self, X, Y):
-        # cell-centre forward metrics
+    def nodes_full(self):
+        """Node coordinates including the ghost nodes, shape dims_full + 1."""
+        return self._XG, self._YG
+
+    @property
+    def cell_center_metrics(self):
+        if self._metrics is None:
+            self._metrics = self._build(self._XG, self._YG)
+        return self._metrics[0]
+
+    @property
+    def edge_metrics(self):
+        if self._metrics is None:
+            self._metrics = self._build(self._XG, self._YG)
+        return self._metrics[1]
+
+    # -- metrics (computed on first use; the solver can build them on the device
+    #    from nodes_full() with the same arithmetic) -------------------------
+    @staticmethod
+    def _forward(X, Y):
         def dxi(A):
             return 0.5 * ((A[1:, :-1] - A[:-1, :-1]) + (A[1:, 1:] - A[:-1, 1:]))
 
@@ -106,8 +127,12 @@ class CurvilinearGrid2D:
             return 0.5 * ((A[:-1, 1:] - A[:-1, :-1]) + (A[1:, 1:] - A[1:, :-1]))
 
         xxi, xeta, yxi, yeta = dxi(X), deta(X), dxi(Y), deta(Y)
-        J = xxi * yeta - xeta * yxi
-        self.cell_center_metrics = SimpleNamespace(
+        return xxi, xeta, yxi, yeta, xxi * yeta - xeta * yxi
+
+    def _build(self, X, Y):
+        # cell-centre forward metrics
+        xxi, xeta, yxi, yeta, J = self._forward(X, Y)
+        cell_center_metrics = SimpleNamespace(
             J=_f(J),
             xi=SimpleNamespace(x1=_f(yeta / J), x2=_f(-xeta / J)),
             eta=SimpleNamespace(x1=_f(-yxi / J), x2=_f(xxi / J)),
@@ -146,10 +171,7 @@ class CurvilinearGrid2D:
                 eta_hat=SimpleNamespace(x1=_f(-y_xi), x2=_f(x_xi)),
             )
 
-        self.edge_metrics = SimpleNamespace(i_half=face(0), j_half=face(1))
-        xc = 0.25 * (X[:-1, :-1] + X[1:, :-1] + X[:-1, 1:] + X[1:, 1:])
-        yc = 0.25 * (Y[:-1, :-1] + Y[1:, :-1] + Y[:-1, 1:] + Y[1:, 1:])
-        self.centroid_coordinates = SimpleNamespace(x=_f(xc), y=_f(yc))
+        return cell_center_metrics, SimpleNamespace(i_half=face(0), j_half=face(1))
 
     # -- accessors used by the solver --------------------------------------
     def coords(self):
@@ -171,7 +193,8 @@ class CurvilinearGrid2D:
     def phys_dims(self):
         """(L, spacing) as src/pseudo_transient/PseudoTransientSolver.jl:122-139."""
         dom = self.iterators.cell.domain
-        ds = float(np.sqrt(np.min(self.cell_center_metrics.J[dom])))
+        J = self._forward(self._XG, self._YG)[4] if self._metrics is None else self.cell_center_metrics.J
+        ds = float(np.sqrt(np.min(J[dom])))
         x, y = self.coords()
         L = min(abs(x.max() - x.min()), abs(y.max() - y.min()))
         return float(L), (ds, ds)
@@ -226,7 +249,55 @@ class CurvilinearGrid3D:
                 domain=(slice(h, h + ni), slice(h, h + nj), slice(h, h + nk)),
             )
         )
-        self._build(G)
+        self._G = G
+        cen = []
+        for c in range(3):
+            A = G[c]
+            acc = 0.0
+            for u in (0, 1):
+                for v in (0, 1):
+                    for w in (0, 1):
+                        acc = acc + A[slice(u, A.shape[0] - 1 + u), slice(v, A.shape[1] - 1 + v), slice(w, A.shape[2] - 1 + w)]
+            cen.append(_f(acc / 8.0))
+        self.centroid_coordinates = SimpleNamespace(x=cen[0], y=cen[1], z=cen[2])
+        self._metrics = None
+
+    def nodes_full(self):
+        """Node coordinates including the ghost nodes, shape dims_full + 1."""
+        return tuple(self._G)
+
+    @property
+    def cell_center_metrics(self):
+        if self._metrics is None:
+            self._metrics = self._build(self._G)
+        return self._metrics[0]
+
+    @property
+    def edge_metrics(self):
+        if self._metrics is None:
+            self._metrics = self._build(self._G)
+        return self._metrics[1]
+
+    @staticmethod
+    def _det_inv(M):
+        """det and inverse of the 3x3 matrices M[..., c, a] by cofactors, in a
+        fixed order of operations (the device kernel evaluates the same)."""
+        m = [[M[..., r, q] for q in range(3)] for r in range(3)]
+        c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1]
+        c01 = m[1][0] * m[2][2] - m[1][2] * m[2][0]
+        c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0]
+        det = m[0][0] * c00 - m[0][1] * c01 + m[0][2] * c02
+        adj = [
+            [c00, m[0][2] * m[2][1] - m[0][1] * m[2][2], m[0][1] * m[1][2] - m[0][2] * m[1][1]],
+            [m[1][2] * m[2][0] - m[1][0] * m[2][2], m[0][0] * m[2][2] - m[0][2] * m[2][0],
+             m[0][2] * m[1][0] - m[0][0] * m[1][2]],
+            [c02, m[0][1] * m[2][0] - m[0][0] * m[2][1], m[0][0] * m[1][1] - m[0][1] * m[1][0]],
+        ]
+        inv = np.empty(M.shape)
+        for a in range(3):
+            for c in range(3):
+                inv[..., a, c] = adj[a][c] / det
+        return det, inv
 
     @staticmethod
     def _dnode(A, axis):
@@ -254,15 +325,14 @@ class CurvilinearGrid3D:
         # D[c][a] = d(x_c)/d(xi_a) at cell centres
         D = [[self._dnode(X[c], a) for a in range(3)] for c in range(3)]
         M = np.stack([np.stack(D[c], axis=-1) for c in range(3)], axis=-2)  # [..., c, a]
-        J = np.linalg.det(M)
-        inv = np.linalg.inv(M)  # [..., a, c] = d(xi_a)/d(x_c)
+        J, inv = self._det_inv(M)  # inv[..., a, c] = d(xi_a)/d(x_c)
         names = ("xi", "eta", "zeta")
         comp = ("x1", "x2", "x3")
-        self.cell_center_metrics = SimpleNamespace(J=_f(J))
+        ccm = SimpleNamespace(J=_f(J))
         for a in range(3):
-            setattr(self.cell_center_metrics, names[a], SimpleNamespace(**{comp[c]: _f(inv[..., a, c]) for c in range(3)}))
+            setattr(ccm, names[a], SimpleNamespace(**{comp[c]: _f(inv[..., a, c]) for c in range(3)}))
         for c in range(3):
-            setattr(self.cell_center_metrics, comp[c], SimpleNamespace(**{names[a]: _f(M[..., c, a]) for a in range(3)}))
+            setattr(ccm, comp[c], SimpleNamespace(**{names[a]: _f(M[..., c, a]) for a in range(3)}))
         shape = self.dims_full
         faces = []
         for b in range(3):
@@ -296,24 +366,13 @@ class CurvilinearGrid3D:
                         hi[o] = lo[o] = slice(1, None) if v else slice(None, -1)
                         acc = acc + (A[tuple(hi)] - A[tuple(lo)])
                     Mf[..., c, a] = 0.5 * acc
-            Jf = np.linalg.det(Mf)
-            invf = np.linalg.inv(Mf)
+            Jf, invf = self._det_inv(Mf)
             hat = invf * Jf[..., None, None]
             ns = SimpleNamespace(J=_f(Jf))
             for a in range(3):
                 setattr(ns, names[a] + "_hat", SimpleNamespace(**{comp[c]: _f(hat[..., a, c]) for c in range(3)}))
             faces.append(ns)
-        self.edge_metrics = SimpleNamespace(i_half=faces[0], j_half=faces[1], k_half=faces[2])
-        cen = []
-        for c in range(3):
-            A = X[c]
-            acc = 0.0
-            for u in (0, 1):
-                for v in (0, 1):
-                    for w in (0, 1):
-                        acc = acc + A[slice(u, A.shape[0] - 1 + u), slice(v, A.shape[1] - 1 + v), slice(w, A.shape[2] - 1 + w)]
-            cen.append(_f(acc / 8.0))
-        self.centroid_coordinates = SimpleNamespace(x=cen[0], y=cen[1], z=cen[2])
+        return ccm, SimpleNamespace(i_half=faces[0], j_half=faces[1], k_half=faces[2])
 
     def coords(self):
         return self._xyz

@@ -216,7 +216,10 @@ _STEP_KW = {
 class PseudoTransientSolver:
     """Device-resident APT solver state (PseudoTransientSolver.jl:28-46)."""
 
-    def __init__(self, mesh, bcs, *, backend=None, mean="harmonic", T=np.float64, **kwargs):
+    def __init__(self, mesh, bcs, *, backend=None, mean="harmonic", T=np.float64, metrics=None, **kwargs):
+        """``metrics``: "device" builds the metric arrays on the GPU from the
+        mesh's ghost-inclusive nodes (``mesh.nodes_full()``, the default when the
+        mesh has them), "host" uploads the mesh's own metric arrays."""
         if T not in (np.float64, float, "Float64"):
             raise TypeError("the drop-in path keeps the reference's Float64 state (see precision= for FP32 sweeps)")
         lib = L.load()
@@ -247,12 +250,25 @@ class PseudoTransientSolver:
             finally:
                 self.close()
         self.L, self.spacing = mesh.phys_dims()
-        cells = [_fortran64(a) for a in mesh.cell_metric_list()]
-        edges = [_fortran64(a) for a in mesh.edge_metric_list()]
-        cp = (L._dp * len(cells))(*[L.ptr(a) for a in cells])
-        ep = (L._dp * len(edges))(*[L.ptr(a) for a in edges])
         sp = (C.c_double * 3)(*(list(self.spacing) + [0.0] * (3 - self.ndim)))
-        _check(lib.cdiff_set_metrics(h, cp, ep, float(self.L), sp), h)
+        if metrics is None:
+            metrics = "device" if hasattr(mesh, "nodes_full") else "host"
+        if metrics == "device":
+            nodes = [_fortran64(a) for a in mesh.nodes_full()]
+            want = tuple(d + 1 for d in self.shape)
+            if any(a.shape != want for a in nodes):
+                raise ValueError(f"node arrays must have shape {want}")
+            npp = (L._dp * len(nodes))(*[L.ptr(a) for a in nodes])
+            _check(lib.cdiff_set_nodes(h, npp, float(self.L), sp), h)
+        elif metrics == "host":
+            cells = [_fortran64(a) for a in mesh.cell_metric_list()]
+            edges = [_fortran64(a) for a in mesh.edge_metric_list()]
+            cp = (L._dp * len(cells))(*[L.ptr(a) for a in cells])
+            ep = (L._dp * len(edges))(*[L.ptr(a) for a in edges])
+            _check(lib.cdiff_set_metrics(h, cp, ep, float(self.L), sp), h)
+        else:
+            raise ValueError('metrics must be "device" or "host"')
+        self.metrics = metrics
         self._rho_id = None
         self.iterators = SimpleNamespace(
             domain=SimpleNamespace(cartesian=mesh.iterators.cell.domain),

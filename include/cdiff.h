@@ -161,6 +161,16 @@ int cdiff_create(int ndim, const int64_t *dims_full, int nhalo, int mean, const 
 int cdiff_set_metrics(cdiff_solver *h, const double *const *cell, const double *const *edge, double L,
                       const double *spacing);
 
+/* The same metrics built on the device from the node coordinates: nodes[c]
+ * (c < ndim) is the ghost-inclusive node grid of coordinate c, shape
+ * dims_full + 1 per axis, first index fastest (the nodes CurvilinearGrids.jl
+ * derives its metric arrays from; consumed at PseudoTransientSolver.jl:122-153,
+ * mesh_metric_cache.jl:39-250, flux_divergence.jl:11-14).  Cell metrics from
+ * the mean node differences of the cell's edges, face metrics from the two
+ * adjacent cells (normal derivative) and the face's own nodes (tangential);
+ * L and spacing as for cdiff_set_metrics. */
+int cdiff_set_nodes(cdiff_solver *h, const double *const *nodes, double L, const double *spacing);
+
 /* The conductivity law and heat capacity of the following steps: the kappa and
  * c_p arguments of nonlinear_thermal_conduction_step! and update_conductivity!
  * (src/nonlinear_thermal_conduction.jl:51-72, src/conductivity.jl:15). */
