@@ -27,6 +27,7 @@ EXPORTS = (
     "cdiff_create",
     "cdiff_set_metrics",
     "cdiff_set_nodes",
+    "cdiff_set_axisymmetric",
     "cdiff_set_kappa",
     "cdiff_set_density",
     "cdiff_set_source",
@@ -126,6 +127,7 @@ def load(path: str = LIB_PATH):
     L.cdiff_create.argtypes = [C.c_int, C.POINTER(C.c_int64), C.c_int, C.c_int, C.POINTER(BC), C.POINTER(Kappa), C.c_double, C.POINTER(_vp)]
     L.cdiff_set_metrics.argtypes = [_vp, C.POINTER(_dp), C.POINTER(_dp), C.c_double, _dp]
     L.cdiff_set_nodes.argtypes = [_vp, C.POINTER(_dp), C.c_double, _dp]
+    L.cdiff_set_axisymmetric.argtypes = [_vp, _dp]
     L.cdiff_set_kappa.argtypes = [_vp, C.POINTER(Kappa), C.c_double]
     L.cdiff_set_density.argtypes = [_vp, _dp]
     L.cdiff_set_source.argtypes = [_vp, _dp]

@@ -292,7 +292,7 @@ static int finish_metrics(cdiff_solver* h, const MetricIn& m, std::vector<double
   double* c[9] = {nullptr};
   for (int t = 0; t < 9; ++t) c[t] = t < (nd == 2 ? 5 : 9) ? h->coef[t] : h->coef[0];
   hipLaunchKernelGGL(k_metric_coefs, grid_full(h->g), dim3(256), 0, h->stream, h->g, m, c[0], c[1], c[2], c[3], c[4],
-                     c[5], c[6], c[7], c[8]);
+                     c[5], c[6], c[7], c[8], (const double*)h->rz_r);
   hipError_t le = hipGetLastError();
   hipError_t se = hipStreamSynchronize(h->stream);
   cleanup();
@@ -340,6 +340,30 @@ int cdiff_set_metrics(cdiff_solver* h, const double* const* cell, const double* 
       m.edge[t - nm] = d;
   }
   return finish_metrics(h, m, tmp, L, spacing);
+}
+
+int cdiff_set_axisymmetric(cdiff_solver* h, const double* r) {
+  if (!h) return CDIFF_BAD_ARG;
+  if (h->ndim != 2) return h->fail(CDIFF_BAD_ARG, "axisymmetric geometry is 2D (x = r, y = z)");
+  if (h->metrics_set) return h->fail(CDIFF_BAD_ARG, "cdiff_set_axisymmetric must precede the metrics");
+  if (!r) {
+    h->rz_r = nullptr;  // buffer stays allocated in `owned` until destroy
+    return CDIFF_OK;
+  }
+  const Geo& g = h->g;
+  for (int j = g.lo[1]; j <= g.hi[1]; ++j)
+    for (int i = g.lo[0]; i <= g.hi[0]; ++i)
+      if (!(r[i + (int64_t)g.n[0] * j] > 0.0))
+        return h->fail(CDIFF_BAD_ARG, "axisymmetric mesh needs r > 0 at every domain cell centre");
+  if (!h->rz_buf) {
+    int rc = alloc_field(h, &h->rz_buf);
+    if (rc) return rc;
+  }
+  int rc = upload(h, h->rz_buf, r);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->rz_r = h->rz_buf;
+  return CDIFF_OK;
 }
 
 int cdiff_set_nodes(cdiff_solver* h, const double* const* nodes, double L, const double* spacing) {

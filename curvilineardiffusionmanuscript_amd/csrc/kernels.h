@@ -23,7 +23,7 @@ struct Coefs {
 };
 
 __global__ void k_metric_coefs(Geo g, MetricIn m, double* out0, double* out1, double* out2, double* out3, double* out4,
-                               double* out5, double* out6, double* out7, double* out8);
+                               double* out5, double* out6, double* out7, double* out8, const double* rz);
 __global__ void k_alpha_full(Geo g, KappaLaw kap, const double* rho, const double* dom_src, const double* halo_src,
                              double* alpha);
 __global__ void k_iter_params_full(Geo g, IterConst ic, const double* rho, const double* alpha, double* dtau,

@@ -29,7 +29,7 @@ __device__ __forceinline__ bool in_domain(const Geo& g, int i, int j, int k) {
 __global__ void k_metric_coefs(Geo g, MetricIn m, double* __restrict__ out0, double* __restrict__ out1,
                                double* __restrict__ out2, double* __restrict__ out3, double* __restrict__ out4,
                                double* __restrict__ out5, double* __restrict__ out6, double* __restrict__ out7,
-                               double* __restrict__ out8) {
+                               double* __restrict__ out8, const double* __restrict__ rz) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t ntot = (int64_t)g.n[0] * g.n[1] * g.n[2];
   if (e >= ntot) return;
@@ -74,8 +74,10 @@ __global__ void k_metric_coefs(Geo g, MetricIn m, double* __restrict__ out0, dou
     out0[c] = xix * xix + xiy * xiy;
     out1[c] = etx * etx + ety * ety;
     out2[c] = 0.25 * (etx * xix + ety * xiy);
-    out3[c] = cache[0];
-    out4[c] = cache[1];
+    // axisymmetric (x = r): (1/r) dT/dr enters like the metric-Laplacian terms,
+    // cache_a + d(xi_a)/dr / r (not part of the reference APT)
+    out3[c] = rz ? cache[0] + xix / rz[c] : cache[0];
+    out4[c] = rz ? cache[1] + etx / rz[c] : cache[1];
   } else {
     const double xix = mm[0][0], xiy = mm[0][1], xiz = mm[0][2];
     const double etx = mm[1][0], ety = mm[1][1], etz = mm[1][2];

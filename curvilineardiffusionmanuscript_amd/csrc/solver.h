@@ -109,6 +109,8 @@ struct cdiff_solver {
   // learned warm start (2D FP32_DC steps)
   cdiff_net* net = nullptr;
   double* jac = nullptr;         // cell Jacobian (2D), for the network's features
+  double* rz_buf = nullptr;      // cell-centre radius (2D axisymmetric)
+  const double* rz_r = nullptr;  // rz_buf when the geometry is axisymmetric, else null
   bool have_prev = false;        // u_prev holds the previous time level T^{n-1}
   double dt_prev = 0.0;
   double* learn_scratch = nullptr;  // >= 1024 doubles: reductions of the feature pass

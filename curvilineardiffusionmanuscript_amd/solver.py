@@ -216,10 +216,13 @@ _STEP_KW = {
 class PseudoTransientSolver:
     """Device-resident APT solver state (PseudoTransientSolver.jl:28-46)."""
 
-    def __init__(self, mesh, bcs, *, backend=None, mean="harmonic", T=np.float64, metrics=None, **kwargs):
+    def __init__(self, mesh, bcs, *, backend=None, mean="harmonic", T=np.float64, metrics=None, axisymmetric=False,
+                 **kwargs):
         """``metrics``: "device" builds the metric arrays on the GPU from the
         mesh's ghost-inclusive nodes (``mesh.nodes_full()``, the default when the
-        mesh has them), "host" uploads the mesh's own metric arrays."""
+        mesh has them), "host" uploads the mesh's own metric arrays.
+        ``axisymmetric``: 2D RZ geometry with x = r, y = z (the operator gains
+        (1/r) dT/dr; r = the cell centroid's x); not in the reference APT."""
         if T not in (np.float64, float, "Float64"):
             raise TypeError("the drop-in path keeps the reference's Float64 state (see precision= for FP32 sweeps)")
         lib = L.load()
@@ -251,6 +254,13 @@ class PseudoTransientSolver:
                 self.close()
         self.L, self.spacing = mesh.phys_dims()
         sp = (C.c_double * 3)(*(list(self.spacing) + [0.0] * (3 - self.ndim)))
+        self.axisymmetric = bool(axisymmetric)
+        if self.axisymmetric:
+            if self.ndim != 2:
+                self.close()
+                raise ValueError("axisymmetric geometry is 2D (x = r, y = z)")
+            r = _fortran64(mesh.centroid_coordinates.x)
+            _check(lib.cdiff_set_axisymmetric(h, L.ptr(r)), h)
         if metrics is None:
             metrics = "device" if hasattr(mesh, "nodes_full") else "host"
         if metrics == "device":

@@ -39,16 +39,22 @@ def _centroids(mesh):
     return (c.x, c.y) if mesh.ndim == 2 else (c.x, c.y, c.z)
 
 
-def gaussian_forcing(mesh, t, center, law, *, fwhm=1.0, damp=4.0, no_t=False, rho=1.0, cp=1.0, source=None):
+def gaussian_forcing(mesh, t, center, law, *, fwhm=1.0, damp=4.0, no_t=False, rho=1.0, cp=1.0, source=None,
+                     axisymmetric=False):
     """(A, B) such that f = A + B*alpha makes the Gaussian exact solution
     satisfy the discrete equations up to truncation error.
 
-    ``law``: a PowerLawConductivity (kappa0 |rho|^a |T|^b)."""
+    ``law``: a PowerLawConductivity (kappa0 |rho|^a |T|^b).  ``axisymmetric``:
+    2D RZ geometry (x = r), whose Laplacian has the extra (1/r) du/dr."""
     X = _centroids(mesh)
     u = gaussian_exact(mesh, t, center, fwhm, damp, no_t)
     d = [-2.0 * (x - c) / fwhm for x, c in zip(X, center)]  # grad u / u
     grad2 = sum(g * g for g in d) * u * u  # |grad u|^2
     lap = u * (sum(g * g for g in d) - 2.0 * mesh.ndim / fwhm)
+    if axisymmetric:
+        if mesh.ndim != 2:
+            raise ValueError("axisymmetric geometry is 2D (x = r, y = z)")
+        lap = lap + d[0] * u / X[0]
     b = law.t_exp
     rho_term = law.kappa0 * (abs(rho) ** law.rho_exp) / (rho * cp)
     with np.errstate(divide="ignore", invalid="ignore"):

@@ -171,6 +171,14 @@ int cdiff_set_metrics(cdiff_solver *h, const double *const *cell, const double *
  * L and spacing as for cdiff_set_metrics. */
 int cdiff_set_nodes(cdiff_solver *h, const double *const *nodes, double L, const double *spacing);
 
+/* 2D axisymmetric (RZ) geometry, x = r, y = z: r is the full-size array of
+ * cell-centre radii (> 0 on the domain), NULL for planar (the default).  The
+ * operator becomes div(alpha grad T) in cylindrical coordinates: the planar
+ * operator plus alpha (1/r) dT/dr.  Must precede cdiff_set_metrics /
+ * cdiff_set_nodes.  No counterpart in the reference APT: its RZ operator
+ * exists only as a derivation (scripts/derivation_noncons_2d_rz.jl:111-126). */
+int cdiff_set_axisymmetric(cdiff_solver *h, const double *r);
+
 /* The conductivity law and heat capacity of the following steps: the kappa and
  * c_p arguments of nonlinear_thermal_conduction_step! and update_conductivity!
  * (src/nonlinear_thermal_conduction.jl:51-72, src/conductivity.jl:15). */

@@ -469,6 +469,15 @@ void ora_update_metric_cache(ora_solver *s) {
           }
           s->cache[a][c] = acc;
         }
+        /* axisymmetric (x = r): div(alpha grad T) gains alpha (1/r) dT/dr with
+         * dT/dr = xi_r T_xi + eta_r T_eta, which enters the divergence exactly
+         * like the metric-Laplacian terms (cache_a times the cell-averaged
+         * flux): cache_a += d(xi_a)/dr / r.  Not part of the reference APT (its
+         * RZ operator exists only as scripts/derivation_noncons_2d_rz.jl). */
+        if (s->rz_r && nd == 2) {
+          s->cache[0][c] = s->cache[0][c] + s->cell[0][c] / s->rz_r[c];
+          s->cache[1][c] = s->cache[1][c] + s->cell[2][c] / s->rz_r[c];
+        }
       }
 }
 

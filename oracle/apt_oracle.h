@@ -38,6 +38,10 @@ typedef struct ora_solver {
   double *tzA, *tzB; /* twilight-zone forcing f = tzA + tzB*alpha, or NULL */
   double *tmp_u;
   char err[512];
+  /* 2D axisymmetric (RZ, x = r): cell-centre radius, or NULL (planar).  Not
+   * in the reference APT; the operator gains (1/r) dT/dr (see
+   * ora_update_metric_cache). */
+  double *rz_r;
 } ora_solver;
 
 typedef struct {

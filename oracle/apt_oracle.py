@@ -64,6 +64,7 @@ class _Solver(C.Structure):
         ("tzB", _dp),
         ("tmp_u", _dp),
         ("err", C.c_char * 512),
+        ("rz_r", _dp),
     ]
 
 
@@ -174,9 +175,12 @@ class OracleError(RuntimeError):
 class OracleSolver:
     """CPU restatement of PseudoTransientSolver for one mesh (2D or 3D)."""
 
-    def __init__(self, mesh, bcs, *, mean="harmonic", kappa=(1.0, 0.0, 0.0), cp=1.0, rho=None, source=None):
+    def __init__(self, mesh, bcs, *, mean="harmonic", kappa=(1.0, 0.0, 0.0), cp=1.0, rho=None, source=None,
+                 axisymmetric=False):
         """``bcs``: sequence of (kind, value) in (ilo, ihi, jlo, jhi[, klo, khi]) order.
-        ``kappa``: (kappa0, t_exp, rho_exp) for kappa = kappa0 |rho|^rho_exp |T|^t_exp."""
+        ``kappa``: (kappa0, t_exp, rho_exp) for kappa = kappa0 |rho|^rho_exp |T|^t_exp.
+        ``axisymmetric``: 2D RZ geometry with x = r (not in the reference APT):
+        the operator gains (1/r) dT/dr, r = the cell centroid's x."""
         L_ = lib()
         nd = mesh.ndim
         dims = mesh.dims_full
@@ -226,6 +230,14 @@ class OracleSolver:
             s.cell[i] = _ptr(a)
         for i, a in enumerate(self.edge):
             s.edge[i] = _ptr(a)
+        self.rz_r = None
+        if axisymmetric:
+            if nd != 2:
+                raise ValueError("axisymmetric geometry is 2D (x = r, y = z)")
+            self.rz_r = np.asfortranarray(mesh.centroid_coordinates.x, dtype=np.float64)
+            if np.min(self.rz_r[mesh.iterators.cell.domain]) <= 0.0:
+                raise ValueError("axisymmetric mesh needs r > 0 at every domain cell centre")
+            s.rz_r = _ptr(self.rz_r)
         self._s = s
         rc = L_.ora_init(C.byref(s))
         if rc:
