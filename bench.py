@@ -189,8 +189,9 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU: this build has no multi-GPU path (the APT would shard by "
-            "domain decomposition with per-iteration halo exchange; see DESIGN.md). Run with --gpus 1.",
+            "bench.py measures one GPU: there is no multi-GPU path (the APT would shard by domain "
+            "decomposition, exchanging a one-cell ring of T and the ring faces' fluxes every "
+            "pseudo-iteration). Run with --gpus 1.",
             file=sys.stderr,
         )
         sys.exit(2)

@@ -3,7 +3,7 @@
 Both sides get the same mesh metrics, initial temperature, BCs and
 conductivity law; after every time step the temperature, the pseudo-iteration
 count, the error estimates and next_dt are compared.  Tolerances: 1e-12
-relative max-norm on T (the north-star bar is 1e-10) and identical iteration
+relative max-norm on T (the required parity is 1e-10) and identical iteration
 counts.
 """
 import numpy as np
