@@ -36,65 +36,102 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, ma
 
 __device__ __forceinline__ float bf(const __bf16 v) { return (float)v; }
 
-constexpr int CONV_ROWS = 4;  // rows per forward block (one wave per row)
+constexpr int CONV_RB = 16;  // output rows per forward block
 
+// 3x3 convolution, CIN -> 32 channels, as an implicit GEMM on MFMA.  A block
+// of 4 waves owns a 64-pixel-wide strip of CONV_RB rows; wave w computes the
+// 16-pixel N-tile w of every row for both 16-channel M-tiles, with the
+// weights (A operands) in registers for the whole strip.  The input rows pass
+// through a 3-row LDS ring (66 pixels x CIN channels): each input row is read
+// from memory once per strip, the next row's load is in flight while the
+// current row is computed, and the nine taps read their B operands (8
+// channels of one pixel per lane) from LDS.  The MFMA chain per output tile
+// is the same as a direct evaluation (taps in order), so results do not
+// depend on the staging.
+//   MODE 0: y = relu(acc + bias)      MODE 1: y = acc where mask > 0, else 0
 template <int CIN, int MODE>
 __global__ __launch_bounds__(256) void k_conv3x3(Img m, const __bf16* __restrict__ X, const __bf16* __restrict__ Wp,
                                                  const float* __restrict__ bias, const __bf16* __restrict__ mask,
                                                  __bf16* __restrict__ Y) {
   constexpr int KB = CIN == 32 ? 9 : 3;  // K-blocks of 32
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int RX = 66;                 // ring row: pixels x0-1 .. x0+64
+  constexpr int CH = CIN / 8;            // 16-byte chunks per pixel
+  constexpr int NCH = RX * CH;           // chunks per ring row
+  constexpr int NLD = (NCH + 255) / 256;
+  __shared__ bf16x8 sRow[3][NCH];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  const int y = blockIdx.y * CONV_ROWS + w;
-  if (y >= m.H) return;  // wave-uniform
-  const int x0 = blockIdx.x * 64;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * CONV_RB;
+  const int y1 = min(y0 + CONV_RB, m.H);  // rows y0 .. y1-1
   bf16x8 a[2][KB];
 #pragma unroll
   for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) a[mt][kb] = load8(Wp + (16 * mt + lr) * (KB * 32) + kb * 32 + 8 * lg);
-  f32x4 acc[4][2];
+
+  // input row yi (-1 .. H, the zero border included) into registers
+  bf16x8 pre[NLD];
+  auto fetch = [&](int yi) {
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt) acc[nt][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int kb = 0; kb < KB; ++kb) {
-    const int tap = CIN == 32 ? kb : 4 * kb + lg;
-    const int c0 = CIN == 32 ? 8 * lg : 0;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    bf16x8 b[4];
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      const int x = x0 + 16 * nt + lr;
-      b[nt] = (x < m.W && tap < 9) ? load8(X + img_pix(m, y + dy, x + dx) * CIN + c0) : zero8();
-    }
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      acc[nt][0] = mfma16(a[0][kb], b[nt], acc[nt][0]);
-      acc[nt][1] = mfma16(a[1][kb], b[nt], acc[nt][1]);
-    }
-  }
-#pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    const int x = x0 + 16 * nt + lr;
-    if (x >= m.W) continue;
-    const int64_t p = img_pix(m, y, x);
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-      const int co = 16 * mt + 4 * lg;
-      bf16x4 o;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float v = acc[nt][mt][r];
-        if (MODE == 0)
-          v = fmaxf(v + bias[co + r], 0.f);
-        else
-          v = bf(mask[p * NET_C + co + r]) > 0.f ? v : 0.f;
-        o[r] = (__bf16)v;
+    for (int q = 0; q < NLD; ++q) {
+      const int t = tid + q * 256;
+      pre[q] = zero8();
+      if (t < NCH) {
+        const int px = t / CH, c = t % CH;
+        const int xi = x0 - 1 + px;
+        if (yi <= m.H && xi <= m.W) pre[q] = load8(X + img_pix(m, yi, xi) * CIN + 8 * c);
       }
-      *reinterpret_cast<bf16x4*>(Y + p * NET_C + co) = o;
     }
+  };
+  auto store = [&](int yi) {
+    bf16x8* R = sRow[(yi + 3) % 3];
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+      const int t = tid + q * 256;
+      if (t < NCH) R[t] = pre[q];
+    }
+  };
+  fetch(y0 - 1);
+  store(y0 - 1);
+  fetch(y0);
+  store(y0);
+  fetch(y0 + 1);
+  const int xo = 16 * w + lr;  // output pixel of this lane within the strip
+  for (int y = y0; y < y1; ++y) {
+    store(y + 1);
+    __syncthreads();  // rows y-1, y, y+1 in LDS
+    if (y + 1 < y1) fetch(y + 2);
+    f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const int tap = CIN == 32 ? kb : 4 * kb + lg;
+      const int c0 = CIN == 32 ? lg : 0;
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      const bf16x8 bv = tap < 9 ? sRow[(y + dy + 3) % 3][(xo + 1 + dx) * CH + c0] : zero8();
+      acc[0] = mfma16(a[0][kb], bv, acc[0]);
+      acc[1] = mfma16(a[1][kb], bv, acc[1]);
+    }
+    const int x = x0 + xo;
+    if (x < m.W) {
+      const int64_t p = img_pix(m, y, x);
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) {
+        const int co = 16 * mt + 4 * lg;
+        bf16x4 o;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float v = acc[mt][r];
+          if (MODE == 0)
+            v = fmaxf(v + bias[co + r], 0.f);
+          else
+            v = bf(mask[p * NET_C + co + r]) > 0.f ? v : 0.f;
+          o[r] = (__bf16)v;
+        }
+        *reinterpret_cast<bf16x4*>(Y + p * NET_C + co) = o;
+      }
+    }
+    __syncthreads();  // row y-1's slot is reloaded next
   }
 }
 
@@ -460,7 +497,7 @@ dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 void launch_conv3x3(int cin, int mode, Img m, const __bf16* X, const __bf16* Wp, const float* bias,
                     const __bf16* mask, __bf16* Y, hipStream_t s) {
-  const dim3 grid((m.W + 63) / 64, (m.H + CONV_ROWS - 1) / CONV_ROWS);
+  const dim3 grid((m.W + 63) / 64, (m.H + CONV_RB - 1) / CONV_RB);
   if (cin == NET_C0)
     hipLaunchKernelGGL((k_conv3x3<NET_C0, 0>), grid, dim3(256), 0, s, m, X, Wp, bias, mask, Y);
   else if (mode == 0)

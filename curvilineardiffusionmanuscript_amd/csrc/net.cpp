@@ -317,6 +317,8 @@ int cdiff_net_set_gate(cdiff_net* n, float gate) {
 int cdiff_set_warm_start(cdiff_solver* h, cdiff_net* n) {
   if (!h) return CDIFF_BAD_ARG;
   h->net = n;
+  h->warm_rejects = 0;
+  h->warm_skip = 0;
   return CDIFF_OK;
 }
 

@@ -369,7 +369,13 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   // learned warm start: features of (T^n, T^{n-1}) and the prediction, before
   // the prologue overwrites u_prev
   warm_pending = false;
-  if (net && o.precision == CDIFF_PREC_FP32_DC && ndim == 2 && lin_ok()) {
+  const bool warm_here = net && (o.precision == CDIFF_PREC_FP32_DC || o.precision == CDIFF_PREC_FP64) && ndim == 2 &&
+                         lin_ok();
+  // after predictions lost to T^n, the network sits out 1, 3, 7 (at most 8)
+  // steps before it is consulted again
+  const bool warm_skipped = warm_here && warm_skip > 0;
+  if (warm_skipped) --warm_skip;
+  if (warm_here && !warm_skipped) {
     if (!learn_ts) {
       HIP_TRY(hipMalloc((void**)&learn_ts, 2 * sizeof(double)));
       owned.push_back(learn_ts);
@@ -386,11 +392,14 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
     hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)Tdev, d);
     LAUNCH_CHECK("k_copy");
   }
-  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, o.rtol, o.atol);
-  hipLaunchKernelGGL(k_validate, grid_full(g), dim3(256), 0, stream, g, (const double*)alpha, o.enforce_positivity, 0,
-                     ctrl);
-  if (S) hipLaunchKernelGGL(k_validate, grid_full(g), dim3(256), 0, stream, g, (const double*)S, 0, 16, ctrl);
-  hipLaunchKernelGGL(k_ctrl_validated, dim3(1), dim3(1), 0, stream, ctrl);
+  auto reset_and_validate = [&]() {
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, o.rtol, o.atol);
+    hipLaunchKernelGGL(k_validate, grid_full(g), dim3(256), 0, stream, g, (const double*)alpha, o.enforce_positivity,
+                       0, ctrl);
+    if (S) hipLaunchKernelGGL(k_validate, grid_full(g), dim3(256), 0, stream, g, (const double*)S, 0, 16, ctrl);
+    hipLaunchKernelGGL(k_ctrl_validated, dim3(1), dim3(1), 0, stream, ctrl);
+  };
+  reset_and_validate();
   LAUNCH_CHECK("prologue");
   if (dirichlet_outer)
     for (int b = 0; b < 2; ++b) {
@@ -402,6 +411,30 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
     int rc = step_fp32_dc(dt, o, Vpdtau, &st);
     if (rc) return rc;
   } else if (o.precision == CDIFF_PREC_FP64) {
+    if (warm_pending) {
+      // learned warm start of the FP64 APT (valid inputs only): the start with
+      // the smaller FP64 residual, prediction or T^n, with its BC ghosts, in
+      // both buffers; then the convergence control starts afresh
+      HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+      if (!ctrl_h[2].done) {
+        int rc = ensure_fp32();
+        if (rc) return rc;
+        dc_Vpdtau = Vpdtau;
+        if ((rc = dc_residual(dt, 0, nullptr, 1, 1, o))) return rc;
+        HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        bool used = false;
+        if (!ctrl_h[2].done && (rc = warm_start_apply(dt, o, &used))) return rc;
+        st.warm_start = used ? 1 : 0;
+        hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)u[1],
+                           u[0]);
+        cur = 1;
+        reset_and_validate();
+        LAUNCH_CHECK("warm start");
+      }
+      warm_pending = false;
+    }
     if (!o.subcycle_conductivity) {
       // iteration-0 alpha and parameters from the BC-applied u (PseudoTransientSolver.jl:254-261)
       for (int t = 0; t < 3; ++t)

@@ -116,6 +116,8 @@ struct cdiff_solver {
   double* learn_scratch = nullptr;  // >= 1024 doubles: reductions of the feature pass
   double* learn_ts = nullptr;       // device: [max |T^n|, dt max |r(T^n)|] of the current step
   bool warm_pending = false;        // a prediction for this step is in the network's workspace
+  int warm_rejects = 0;             // consecutive steps whose prediction lost to T^n
+  int warm_skip = 0;                // steps left before the network is consulted again
 
   std::string err;
 

@@ -90,8 +90,11 @@ int cdiff_solver::warm_start_apply(double dt, const cdiff_opts& o, bool* used) {
     std::fprintf(stderr, "[dc] residual of the predicted start %.3e (of T^n %.3e)\n", ctrl_h[2].abs_, r0);
   if (ctrl_h[2].abs_ < r0 && ctrl_h[2].status == CDIFF_OK) {
     *used = true;
+    warm_rejects = 0;
     return CDIFF_OK;
   }
+  ++warm_rejects;
+  warm_skip = std::min((1 << std::min(warm_rejects, 4)) - 1, 8);
   // fallback: start from T^n (in u_prev) again
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, o.rtol, o.atol);
   hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)u_prev, u[0]);

@@ -275,7 +275,7 @@ def evaluate(net, problem: str, n: int, steps: int, dt0: float, seed: int = 0, a
         law = S.PowerLawConductivity(1.0, float(rng.uniform(1.0, 3.0)), 0.0)
         bcs = [S.NeumannBC()] * 4
     out = {}
-    for name, prec, warm in (("fp64", "fp64", False), ("fp32_dc", "fp32_dc", False),
+    for name, prec, warm in (("fp64", "fp64", False), ("fp64+learned", "fp64", True), ("fp32_dc", "fp32_dc", False),
                              ("fp32_dc+learned", "fp32_dc", True)):
         sol = S.PseudoTransientSolver(mesh, bcs)
         if warm:

@@ -61,8 +61,10 @@ def test_training_on_captured_pairs_reduces_loss():
     net.close()
 
 
-def test_warm_started_step_reaches_the_same_tolerance():
-    # train briefly, then step a held-out problem with and without the warm start
+@pytest.mark.parametrize("precision", ["fp32_dc", "fp64"])
+def test_warm_started_step_reaches_the_same_tolerance(precision):
+    # train briefly, then step a held-out problem with and without the warm
+    # start (mixed-precision and FP64 steps)
     n, full = 64, 512
     net = LW.WarmStartNet(seed=2)
     net.dataset(n, n, 24)
@@ -80,7 +82,7 @@ def test_warm_started_step_reaches_the_same_tolerance():
         sol.set_temperature(T0)
         dt, its, used, errs = 2e-5, [], [], []
         for _ in range(6):
-            st, dt = sol.step_resident(dt, None, 1.0, law, rtol=0.0, atol=1e-8, max_iter=10**6, precision="fp32_dc")
+            st, dt = sol.step_resident(dt, None, 1.0, law, rtol=0.0, atol=1e-8, max_iter=10**6, precision=precision)
             its.append(st.niter)
             used.append(st.warm_start)
             errs.append(st.abs_err)
