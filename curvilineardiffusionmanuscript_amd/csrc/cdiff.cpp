@@ -192,6 +192,11 @@ int cdiff_create(int ndim, const int64_t* dims_full, int nhalo, int mean, const 
   if (mean != CDIFF_MEAN_HARMONIC && mean != CDIFF_MEAN_ARITHMETIC) return CDIFF_BAD_ARG;
   cdiff_solver* h = new cdiff_solver();
   HIP_TRY(hipGetDevice(&h->device));
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && ncu > 0)
+      h->ncu = ncu;
+  }
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   h->ndim = ndim;
   h->nhalo = nhalo;

@@ -134,7 +134,10 @@ int cdiff_solver::ensure_fp32() {
     // 3D: one iteration per launch; 64 x 8 columns over chunks of planes
     lin_k = 1;
     const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1, nk = g.hi[2] - g.lo[2] + 1;
-    lin3_ch = std::max(1, std::min(nk, env_int("CDIFF_LIN3_CH", 32)));
+    // about two 512-thread blocks per CU (256^3: 64 planes per chunk)
+    const int64_t cols = (int64_t)((ni + 63) / 64) * ((nj + 7) / 8);
+    const int dflt = (int)std::min<int64_t>(256, std::max<int64_t>(4, (cols * nk + 2 * ncu - 1) / (2 * ncu)));
+    lin3_ch = std::max(1, std::min(nk, env_int("CDIFF_LIN3_CH", dflt)));
     lin3_strips = (ni + 63) / 64;
     lin_chunks = (nk + lin3_ch - 1) / lin3_ch;
     lin_tiles = lin3_strips * ((nj + 7) / 8) * lin_chunks;

@@ -83,6 +83,7 @@ struct cdiff_solver {
   // sweep decomposition
   bool tile2d = true;  // 2D: LDS-tiled kernel (else the row-marching kernel)
   bool col3d = true;   // 3D: 64 x 8 column kernel (else one lane per column with lane/wave rings)
+  int ncu = 256;       // compute units of the device
   int strips = 0, chunks = 0, ch = 0, ntiles = 0, nblocks = 0;
   int strips3[2] = {0, 0};
 
