@@ -307,7 +307,7 @@ int cdiff_solver::lin_coefs(double dt, double Vpdtau, int buf) {
   return CDIFF_OK;
 }
 
-int cdiff_solver::launch_lin(double dt, int src, bool check, int nit) {
+int cdiff_solver::launch_lin(double dt, int src, bool check, int nit, bool zero_src) {
   if (!lin || !lin_coefs_ready) return fail(CDIFF_BAD_ARG, "coefficient-form sweep not set up (run an FP32 step first)");
   if (nit < 1 || nit > lin_k) return fail(CDIFF_BAD_ARG, "iterations per launch out of range");
   const int dst = 1 - src;
@@ -319,6 +319,7 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit) {
     B.stripsI = lin3_strips;
     B.ch = lin3_ch;
     B.inv_dt = (float)(1.0 / dt);
+    B.zero_src = zero_src;
     B.e_src = u32[src];
     B.e_dst = u32[dst];
     for (int a = 0; a < 3; ++a) {
@@ -342,6 +343,7 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit) {
   A.ntiles = lin_tiles;
   A.chunks = lin_chunks;
   A.nit = nit;
+  A.zero_src = zero_src;
   A.inv_dt = (float)(1.0 / dt);
   A.e_src = u32[src];
   A.e_dst = u32[dst];
@@ -532,10 +534,14 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
       ctrl_h[2].status = CDIFF_MAX_ITER;
       break;
     }
-    // inner FP32 solve for the correction
-    hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, u32[0]);
-    for (int ax = 0; ax < ndim; ++ax)
-      for (int b = 0; b < 2; ++b) hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, q32[ax][b]);
+    // inner FP32 solve for the correction, from e = 0 and q = 0 (the
+    // coefficient-form sweeps read no state on their first launch)
+    if (!lin) {
+      hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, u32[0]);
+      for (int ax = 0; ax < ndim; ++ax)
+        for (int b = 0; b < 2; ++b)
+          hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, q32[ax][b]);
+    }
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl_in, 0.0, 0.0);
     LAUNCH_CHECK("inner reset");
     // (the coefficient-form sweep's coefficients come with the residual pass)
@@ -554,7 +560,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
         if (lin) {
           // up to lin_nit iterations; a check iteration ends the launch
           while (tl < e && tl - t + 1 < lin_nit && !is_chk(tl)) ++tl;
-          if ((rc = launch_lin(dt, ebuf, is_chk(tl), tl - t + 1))) return rc;
+          if ((rc = launch_lin(dt, ebuf, is_chk(tl), tl - t + 1, t == 1))) return rc;
         } else {
           const bool pair = pairs && !is_chk(t) && t + 1 <= e;
           tl = pair ? t + 1 : t;

@@ -140,7 +140,7 @@ struct cdiff_solver {
   int step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cdiff_stats* stats);
   int launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check, bool pair = false);
   int launch_sweep32(double dt, double Vpdtau, int src, bool check, bool pair = false);
-  int launch_lin(double dt, int src, bool check, int nit);
+  int launch_lin(double dt, int src, bool check, int nit, bool zero_src = false);
   int lin_coefs(double dt, double Vpdtau, int buf);
   bool pairs_ok() const;
   int bench_sweeps(double dt, int precision, int check, int n, double* mean_ms);

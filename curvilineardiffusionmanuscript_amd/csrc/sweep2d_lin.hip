@@ -81,7 +81,7 @@ __global__ __launch_bounds__(LNTHR) void sweep2d_lin_kernel(Lin2DArgs A) {
       const bool yon = i >= g.lo[0] && i <= g.hi[0] && r >= g.lo[1] - 1 && r <= g.hi[1];
       const int64_t c = at(g, i, r, 0);
       if (dom) {
-        e[p] = A.e_src[c];
+        if (!A.zero_src) e[p] = A.e_src[c];
         rb[p] = A.base[c];
         cc[p] = A.cc[c];
         m0[p] = A.m[0][c];
@@ -91,13 +91,13 @@ __global__ __launch_bounds__(LNTHR) void sweep2d_lin_kernel(Lin2DArgs A) {
         m4[p] = A.m[4][c];
       }
       if (xon) {
-        qx[p] = A.qx_src[c];
+        if (!A.zero_src) qx[p] = A.qx_src[c];
         px[p] = A.fx[0][c];
         mx[p] = A.fx[1][c];
         wx[p] = A.fx[2][c];
       }
       if (yon) {
-        qy[p] = A.qy_src[c];
+        if (!A.zero_src) qy[p] = A.qy_src[c];
         py[p] = A.fy[0][c];
         my[p] = A.fy[1][c];
         wy[p] = A.fy[2][c];

@@ -77,27 +77,27 @@ __global__ __launch_bounds__(L3T) void sweep3d_lin_kernel(Lin3DArgs A) {
       for (int ax = 0; ax < 3; ++ax) F[p].f[ax][0] = F[p].f[ax][1] = F[p].f[ax][2] = F[p].f[ax][3] = 0.f;
       if (!rv[p]) continue;
       const int i = gx + rlx[p], j = gy + rly[p];
-      if (in_dom(i, j, k)) F[p].e = A.e_src[at(g, i, j, k)];
+      if (!A.zero_src && in_dom(i, j, k)) F[p].e = A.e_src[at(g, i, j, k)];
       if (rlx[p] < R3X - 1 && face_on(0, i, j, k)) {
         const int64_t c = at(g, i, j, k);
         F[p].f[0][0] = A.f[0 * A.stride + c];
         F[p].f[0][1] = A.f[1 * A.stride + c];
         F[p].f[0][2] = A.f[2 * A.stride + c];
-        F[p].f[0][3] = A.q_src[0][c];
+        F[p].f[0][3] = A.zero_src ? 0.f : A.q_src[0][c];
       }
       if (rly[p] < R3Y - 1 && face_on(1, i, j, k)) {
         const int64_t c = at(g, i, j, k);
         F[p].f[1][0] = A.f[3 * A.stride + c];
         F[p].f[1][1] = A.f[4 * A.stride + c];
         F[p].f[1][2] = A.f[5 * A.stride + c];
-        F[p].f[1][3] = A.q_src[1][c];
+        F[p].f[1][3] = A.zero_src ? 0.f : A.q_src[1][c];
       }
       if (zf && face_on(2, i, j, k - 1)) {
         const int64_t c = at(g, i, j, k - 1);
         F[p].f[2][0] = A.f[6 * A.stride + c];
         F[p].f[2][1] = A.f[7 * A.stride + c];
         F[p].f[2][2] = A.f[8 * A.stride + c];
-        F[p].f[2][3] = A.q_src[2][c];
+        F[p].f[2][3] = A.zero_src ? 0.f : A.q_src[2][c];
       }
     }
   };

@@ -109,6 +109,7 @@ struct Lin2DArgs {
   Geo g;
   int ntiles, chunks;  // tiles of 64 x lin2d_rows(kmax) cells; chunks along j
   int nit;             // iterations of this launch, 1..kmax
+  int zero_src;        // the correction and its fluxes start from 0 (e_src, q*_src not read)
   float inv_dt;
   const float* __restrict__ e_src;
   float* __restrict__ e_dst;
@@ -135,6 +136,7 @@ void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, bool check, int nblocks, h
 struct Lin3DArgs {
   Geo g;
   int ntiles, chunks, stripsI, ch;
+  int zero_src;  // the correction and its fluxes start from 0 (e_src, q_src not read)
   float inv_dt;
   const float* __restrict__ e_src;
   float* __restrict__ e_dst;
