@@ -116,9 +116,8 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
   __syncthreads();
 
   // ---- faces: q' for the residual; P, M, W of the tile's faces
-  auto coefs = [&](int c0, int c1, bool lo_face, bool hi_face, int flo, int fhi, float* const* F, int64_t gc) {
-    const double a0 = sA[c0], a1 = sA[c1];
-    const double abar = face_mean<double>(harm, a0, a1);
+  auto coefs = [&](int c0, int c1, double a0, double a1, double abar, bool lo_face, bool hi_face, int flo, int fhi,
+                   float* const* F, int64_t gc) {
     const double tbar = face_mean<double>(harm, sT[c0], sT[c1]);
     double P = abar, M = abar;
     if (A.newton) {
@@ -152,9 +151,10 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
       double q = 0.0;
       if (i >= g.lo[0] - 1 && i <= g.hi[0] && r >= g.lo[1] && r <= g.hi[1]) {
         const int c0 = cy * OCX + fx, c1 = c0 + 1;
-        q = -face_mean<double>(harm, sA[c0], sA[c1]) * (sU[c1] - sU[c0]);
+        const double a0 = sA[c0], a1 = sA[c1], abar = face_mean<double>(harm, a0, a1);
+        q = -abar * (sU[c1] - sU[c0]);
         if (cy >= 1 && cy <= OTY && (fx >= 1 || i == g.lo[0] - 1))
-          coefs(c0, c1, i == g.lo[0] - 1, i == g.hi[0], 0, 1, A.fx, at(g, i, r, 0));
+          coefs(c0, c1, a0, a1, abar, i == g.lo[0] - 1, i == g.hi[0], 0, 1, A.fx, at(g, i, r, 0));
       }
       sX[t] = q;
     } else if (t < ONFX + ONFY) {
@@ -164,9 +164,10 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
       double q = 0.0;
       if (i >= g.lo[0] && i <= g.hi[0] && r >= g.lo[1] - 1 && r <= g.hi[1]) {
         const int c0 = fy * OCX + fx, c1 = c0 + OCX;
-        q = -face_mean<double>(harm, sA[c0], sA[c1]) * (sU[c1] - sU[c0]);
+        const double a0 = sA[c0], a1 = sA[c1], abar = face_mean<double>(harm, a0, a1);
+        q = -abar * (sU[c1] - sU[c0]);
         if (fx >= 1 && fx <= OTX && (fy >= 1 || r == g.lo[1] - 1))
-          coefs(c0, c1, r == g.lo[1] - 1, r == g.hi[1], 2, 3, A.fy, at(g, i, r, 0));
+          coefs(c0, c1, a0, a1, abar, r == g.lo[1] - 1, r == g.hi[1], 2, 3, A.fy, at(g, i, r, 0));
       }
       sY[u] = q;
     }
