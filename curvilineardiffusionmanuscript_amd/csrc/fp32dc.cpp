@@ -131,8 +131,8 @@ int cdiff_solver::ensure_fp32() {
     lin_chunks = (nj + rows - 1) / rows;
     lin_tiles = ((ni + 63) / 64) * lin_chunks;
   } else if (lin) {
-    // 3D: one iteration per launch; 64 x 8 columns over chunks of planes
-    lin_k = 1;
+    // 3D: one or two iterations per launch; 64 x 8 columns over chunks of planes
+    lin_k = std::max(1, std::min(2, env_int("CDIFF_LIN3_K", 2)));
     const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1, nk = g.hi[2] - g.lo[2] + 1;
     // about two 512-thread blocks per CU (256^3: 64 planes per chunk)
     const int64_t cols = (int64_t)((ni + 63) / 64) * ((nj + 7) / 8);
@@ -141,9 +141,16 @@ int cdiff_solver::ensure_fp32() {
     lin3_strips = (ni + 63) / 64;
     lin_chunks = (nk + lin3_ch - 1) / lin3_ch;
     lin_tiles = lin3_strips * ((nj + 7) / 8) * lin_chunks;
+    // the two-iteration sweep runs one 832-thread block per CU (256^3: 128 planes per chunk)
+    const int dflt2 = (int)std::min<int64_t>(256, std::max<int64_t>(4, (cols * nk + ncu - 1) / ncu));
+    lin3b_ch = std::max(1, std::min(nk, env_int("CDIFF_LIN3B_CH", dflt2)));
+    lin3b_chunks = (nk + lin3b_ch - 1) / lin3b_ch;
+    lin3b_tiles = lin3_strips * ((nj + 7) / 8) * lin3b_chunks;
+    lin3b_nblocks = (lin3b_tiles + 7) / 8 * 8;
   }
   if (lin) {
     lin_nblocks = (lin_tiles + 7) / 8 * 8;
+    const int nparts_need = std::max(lin_nblocks, lin3b_nblocks);
     // one block each: the folded metrics m[c] at m[0] + c * total; the face
     // coefficients [axis][P, M, W] at fx[0] + (3 axis + c) * total
     auto alloc_block = [&](float** p, int n) -> int {
@@ -170,12 +177,12 @@ int cdiff_solver::ensure_fp32() {
       }
     }
     if ((rc = alloc32(&lin_cc))) return rc;
-    if (lin_nblocks > lin_nparts) {
-      HIP_TRY(hipMalloc((void**)&lin_psum, sizeof(double) * lin_nblocks));
+    if (nparts_need > lin_nparts) {
+      HIP_TRY(hipMalloc((void**)&lin_psum, sizeof(double) * nparts_need));
       owned.push_back(lin_psum);
-      HIP_TRY(hipMalloc((void**)&lin_pmax, sizeof(double) * lin_nblocks));
+      HIP_TRY(hipMalloc((void**)&lin_pmax, sizeof(double) * nparts_need));
       owned.push_back(lin_pmax);
-      lin_nparts = lin_nblocks;
+      lin_nparts = nparts_need;
     }
     if (!lin_metrics_valid) {
       if (ndim == 2) {
@@ -314,10 +321,11 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit, bool zero_
   if (ndim == 3) {
     Lin3DArgs B{};
     B.g = g;
-    B.ntiles = lin_tiles;
-    B.chunks = lin_chunks;
+    const bool two = nit == 2;
+    B.ntiles = two ? lin3b_tiles : lin_tiles;
+    B.chunks = two ? lin3b_chunks : lin_chunks;
     B.stripsI = lin3_strips;
-    B.ch = lin3_ch;
+    B.ch = two ? lin3b_ch : lin3_ch;
     B.inv_dt = (float)(1.0 / dt);
     B.zero_src = zero_src;
     B.e_src = u32[src];
@@ -334,7 +342,13 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit, bool zero_
     B.part_sum = lin_psum;
     B.part_max = lin_pmax;
     B.ctrl = ctrl_in;
-    launch_sweep3d_lin(B, check, lin_nblocks, stream);
+    if (two) {
+      launch_sweep3d_lin2(B, check, lin3b_nblocks, stream);
+      lin_last_parts = lin3b_nblocks;
+    } else {
+      launch_sweep3d_lin(B, check, lin_nblocks, stream);
+      lin_last_parts = lin_nblocks;
+    }
     LAUNCH_CHECK("sweep3d_lin");
     return CDIFF_OK;
   }
@@ -362,6 +376,7 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit, bool zero_
   A.part_max = lin_pmax;
   A.ctrl = ctrl_in;
   launch_sweep2d_lin(A, lin_k, check, lin_nblocks, stream);
+  lin_last_parts = lin_nblocks;
   LAUNCH_CHECK("sweep2d_lin");
   return CDIFF_OK;
 }
@@ -571,7 +586,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
         ends.push_back({tl, ebuf});
         if (chk_last) {
           hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(1024), 0, stream, ctrl_in, (const Ctrl*)ctrl,
-                             (const double*)(lin ? lin_psum : part_sum), lin ? lin_nblocks : nparts, N_full,
+                             (const double*)(lin ? lin_psum : part_sum), lin ? lin_last_parts : nparts, N_full,
                              fixed_inner > 0 ? 0.0 : eta, tl, cap, fixed_inner > 0 ? 0.0 : floor_frac, o.rtol,
                              o.atol);
           LAUNCH_CHECK("k_inner_finalize");

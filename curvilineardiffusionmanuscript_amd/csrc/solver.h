@@ -71,6 +71,9 @@ struct cdiff_solver {
   float* lin_fy[3] = {nullptr, nullptr, nullptr};
   float* lin_fz[3] = {nullptr, nullptr, nullptr};
   int lin3_ch = 0, lin3_strips = 0;  // 3D: planes per chunk, 64-wide strips
+  // 3D two-iteration sweep: planes per chunk, chunks, tiles, blocks (one block per CU)
+  int lin3b_ch = 0, lin3b_chunks = 0, lin3b_tiles = 0, lin3b_nblocks = 0;
+  int lin_last_parts = 0;  // partial sums written by the last correction-sweep launch
   float* lin_cc = nullptr;
   bool lin_metrics_valid = false;
   bool lin_coefs_ready = false;

@@ -152,5 +152,7 @@ struct Lin3DArgs {
   const Ctrl* ctrl;
 };
 void launch_sweep3d_lin(const Lin3DArgs& A, bool check, int nblocks, hipStream_t s);
+// two iterations per launch on 64 x 8 columns (sweep3d_lin2.hip)
+void launch_sweep3d_lin2(const Lin3DArgs& A, bool check, int nblocks, hipStream_t s);
 
 }  // namespace cdiff

@@ -125,6 +125,37 @@ def test_fused_3d_outer_pass_matches_separate_kernels(ch, monkeypatch):
     np.testing.assert_allclose(out["1"][0], out["0"][0], rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("ch,chk", [("11", "2"), ("4", "3"), ("1", "4"), ("3", "5")])
+def test_lin3_two_iteration_launch_matches_one(ch, chk, monkeypatch):
+    # the 3D correction sweep with two iterations per launch (two-cell ring,
+    # k-marching pipeline) against one iteration per launch: the same FP32
+    # expressions in the same order, so the same bits.  70 x 13 x 11 cells:
+    # a ragged second strip and row; ch = planes per chunk (one chunk, ragged
+    # chunks, one plane per chunk); chk = sweeps per convergence check (odd
+    # values mix one- and two-iteration launches, and checks land on both)
+    mesh, T0, bd = _mesh_3d_dirichlet()
+    law = S.PowerLawConductivity(1.0, 2.0, 0.0)
+    monkeypatch.setenv("CDIFF_LIN3B_CH", ch)
+    monkeypatch.setenv("CDIFF_DC_CHECK", chk)
+    out = {}
+    for nit in ("2", "1"):
+        monkeypatch.setenv("CDIFF_LIN_NIT", nit)
+        dev = S.PseudoTransientSolver(mesh, bd)
+        T = T0.copy(order="F")
+        S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
+        stats = []
+        for _ in range(2):
+            st, _ = S.nonlinear_thermal_conduction_step(dev, mesh, T, 1.0, 1.0, law, 2e-4, rtol=0.0, atol=1e-8,
+                                                        max_iter=10**6, precision="fp32_dc",
+                                                        calculate_next_dt=False, show_convergence=False)
+            stats.append((st.outer, st.niter, st.abs_err))
+        out[nit] = (T, stats)
+        dev.close()
+    print(out["2"][1], out["1"][1])
+    assert out["2"][1] == out["1"][1]
+    np.testing.assert_array_equal(out["2"][0], out["1"][0])
+
+
 def test_dc_then_fp64_step_continues():
     mesh, T0 = gaussian_2d(64)
     law = S.PowerLawConductivity(1.0)
