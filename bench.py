@@ -14,7 +14,11 @@ kappa = T^3, 8 seeded Gaussian spots on T = 0.01, zero-flux faces, dt0 = 1e-5.
 
 Timed in the same run: the headline method (FP32 correction sweeps inside
 the FP64 defect-correction loop) and the plain FP64 APT, from the same
-state; `value` is the headline method's s/time-step.
+state; `value` is the headline method's s/time-step.  In 2D the same
+mixed-precision steps also run with the learned warm start attached (the
+committed network; config.learned_warm_start): each step starts from the
+network's prediction when that has the smaller FP64 residual, and `value`
+stays the step without the network, the faster of the two on this workload.
 
 Prints one JSON line: s/time-step (lower is better), with
   roofline      the dominant kernel's algorithmic bytes per launch (cells x
@@ -72,7 +76,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(args, mesh, T0, method):
+def run_gpu(args, mesh, T0, method, net=None):
+    """Warm-up steps, then args.steps timed steps of `method` from T0; with
+    `net`, the learned warm start is attached (its back-off state reset with
+    the solver's)."""
     from curvilineardiffusionmanuscript_amd import solver as S
 
     w = WORKLOADS[args.workload]
@@ -82,6 +89,8 @@ def run_gpu(args, mesh, T0, method):
 
     def reset():
         sol.reset_state()
+        if net is not None:
+            net.attach(sol)
         S.update_conductivity(sol, mesh, T0, 1.0, 1.0, law)
         sol.set_temperature(T0)
 
@@ -91,13 +100,14 @@ def run_gpu(args, mesh, T0, method):
         st, dt = sol.step_resident(dt, None, 1.0, law, **kw)
     reset()
     dt = args.dt0
-    iters, checks, steps, outer = [], 0, [], []
+    iters, checks, steps, outer, warm = [], 0, [], [], 0
     t0 = time.perf_counter()  # step_resident returns after the step's work has finished
     for k in range(args.steps):
         st, dt_next = sol.step_resident(dt, None, 1.0, law, **kw)
         iters.append(st.niter)
         outer.append(st.outer)
         checks += st.nchecks
+        warm += int(st.warm_start)
         steps.append((dt, st.niter, st.abs_err, st.time))
         dt = dt_next
     elapsed = time.perf_counter() - t0
@@ -105,11 +115,11 @@ def run_gpu(args, mesh, T0, method):
         log(f"{method} step {k}: dt={d:.3e} niter={it} abs_err={err:.3e} time={tm * 1e3:.2f} ms")
     T_end = sol.get_temperature()
     mean_ms = float("nan")
-    if args.sweep_samples > 0:
+    if args.sweep_samples > 0 and net is None:
         mean_ms = sol_bench(sol, steps[-1][0], method, args.sweep_samples)
     sol.close()
     return dict(elapsed=elapsed, iters=iters, outer=outer, checks=checks, T_end=T_end, sweep_ms=mean_ms,
-                abs_err=[s[2] for s in steps])
+                abs_err=[s[2] for s in steps], warm_used=warm)
 
 
 # dominant kernel of each method and dimension, with the pseudo-iterations
@@ -183,6 +193,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--method", default="fp32_dc", choices=["fp64", "fp32_dc"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 APT comparison run")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the run with the learned warm start (2D mixed precision only)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-samples", type=int, default=200)
@@ -211,6 +223,13 @@ def main():
     ref = None
     if args.method != "fp64" and not args.no_fp64:
         ref = run_gpu(args, mesh, T0, "fp64")
+    lrn = None
+    if args.method == "fp32_dc" and ndim == 2 and not args.no_learned:
+        from curvilineardiffusionmanuscript_amd.learned import WarmStartNet
+
+        net = WarmStartNet.load()
+        lrn = run_gpu(args, mesh, T0, args.method, net=net)
+        net.close()
     n_cells = args.n**ndim
     bpc = BYTES_PER_CELL[(args.method, ndim)]
     s_per_step = g["elapsed"] / args.steps
@@ -231,6 +250,20 @@ def main():
             "speedup_vs_fp64": round(s64 / s_per_step, 3),
             "max_rel_diff_vs_fp64": err,
         }
+    if lrn is not None:
+        sl = lrn["elapsed"] / args.steps
+        log(f"GPU {args.method}+learned: {sl * 1e3:.3f} ms/step, {np.mean(lrn['iters']):.1f} its/step, warm start "
+            f"used on {lrn['warm_used']} of {args.steps} steps")
+        extra["learned_warm_start"] = {
+            "s_per_step": round(sl, 6),
+            "iters_per_step": round(float(np.mean(lrn["iters"])), 2),
+            "warm_start_used_steps": lrn["warm_used"],
+            "max_abs_err": max(lrn["abs_err"]),
+        }
+        if ref is not None:
+            extra["learned_warm_start"]["speedup_vs_fp64"] = round(s64 / sl, 3)
+            extra["learned_warm_start"]["max_rel_diff_vs_fp64"] = float(
+                np.max(np.abs(lrn["T_end"] - ref["T_end"]))) / scale
 
     kname, per_launch, _ = KERNELS[(args.method, ndim)]
     per_it_ms = g["sweep_ms"]

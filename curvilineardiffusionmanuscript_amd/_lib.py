@@ -58,6 +58,7 @@ EXPORTS = (
     "cdiff_net_train",
     "cdiff_net_eval",
     "cdiff_net_loss_grad",
+    "cdiff_net_predict",
     "cdiff_net_last_error",
     "cdiff_set_warm_start",
     "cdiff_net_set_gate",
@@ -161,6 +162,7 @@ def load(path: str = LIB_PATH):
     L.cdiff_net_train.argtypes = [_vp, C.c_int, C.c_int, C.c_float, C.c_uint64, _dp]
     L.cdiff_net_eval.argtypes = [_vp, C.c_int, _fp, _dp]
     L.cdiff_net_loss_grad.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_uint16), _fp, _fp, _fp, _dp]
+    L.cdiff_net_predict.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_uint16), _fp]
     L.cdiff_net_last_error.argtypes = [_vp]
     L.cdiff_net_last_error.restype = C.c_char_p
     L.cdiff_set_warm_start.argtypes = [_vp, _vp]

@@ -56,6 +56,9 @@ void launch_conv3x3(int cin, int mode, Img m, const __bf16* X, const __bf16* Wp,
 // (|feature 5| > gate in the 3x3 neighbourhood of X0), 0 elsewhere.  With
 // target: also g = 2 (delta - target) / (H W) on active pixels into G4 (FP32,
 // bordered image) and per-block partial sums of (delta - target)^2.
+// inference head on MFMA (no loss); matches launch_head_fwd up to FP32 summation order
+void launch_head_infer(Img m, const __bf16* X3, const __bf16* w4, const float* params, const __bf16* X0, float gate,
+                       float* delta, hipStream_t s);
 void launch_head_fwd(Img m, const __bf16* X3, const __bf16* w4, const float* params, const __bf16* X0, float gate,
                      float* delta, const float* target, float* G4, double* loss_part, int* nparts, hipStream_t s);
 // G3pre = (sum_tap G4(p - off) W4[tap]) * (X3 > 0)

@@ -11,7 +11,9 @@
 //                             bias gradient is one more column with X = 1
 // MFMA operand layout (16x16x32): lane l holds A[l%16][8(l/16) + j],
 // B[8(l/16) + j][l%16] (j = 0..7) and receives D[4(l/16) + r][l%16].
-// The 1-channel head (288 MACs per pixel) runs on the vector ALUs.
+// The 1-channel head (288 MACs per pixel) runs on MFMA for inference (one
+// M-tile, row 0 real) and on the vector ALUs in training, where it also forms
+// the loss gradient.
 #include <cmath>
 
 #include "learn.h"
@@ -187,6 +189,73 @@ __global__ __launch_bounds__(256) void k_head_fwd(Img m, const __bf16* __restric
     double s = 0.0;
     for (int i = 0; i < 256 / WAVE; ++i) s += sred[i];
     loss_part[blockIdx.x] = s;
+  }
+}
+
+// Head forward for inference on MFMA: the 1-channel 3x3 convolution as the
+// implicit GEMM of k_conv3x3 with one 16-row M-tile whose row 0 holds the head
+// weights (rows 1-15 zero), the same strip / LDS-ring structure, and the bias
+// and activity gate applied to row 0.  Lanes 0-15 of each wave hold row 0 of
+// their 16 pixels.  (Training keeps k_head_fwd, which also forms the loss
+// gradient; the two differ only in the FP32 summation order.)
+__global__ __launch_bounds__(256) void k_head_mfma(Img m, const __bf16* __restrict__ X3, const __bf16* __restrict__ w4,
+                                                   const float* __restrict__ params, const __bf16* __restrict__ X0,
+                                                   float gate, float* __restrict__ delta) {
+  constexpr int KB = 9, RX = 66, CH = NET_C / 8, NCH = RX * CH, NLD = (NCH + 255) / 256;
+  __shared__ bf16x8 sRow[3][NCH];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * CONV_RB;
+  const int y1 = min(y0 + CONV_RB, m.H);
+  bf16x8 a[KB];
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) a[kb] = lr == 0 ? load8(w4 + kb * NET_C + 8 * lg) : zero8();
+  const float b4 = params[NP_B4];
+
+  bf16x8 pre[NLD];
+  auto fetch = [&](int yi) {
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+      const int t = tid + q * 256;
+      pre[q] = zero8();
+      if (t < NCH) {
+        const int px = t / CH, c = t % CH;
+        const int xi = x0 - 1 + px;
+        if (yi <= m.H && xi <= m.W) pre[q] = load8(X3 + img_pix(m, yi, xi) * NET_C + 8 * c);
+      }
+    }
+  };
+  auto store = [&](int yi) {
+    bf16x8* R = sRow[(yi + 3) % 3];
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+      const int t = tid + q * 256;
+      if (t < NCH) R[t] = pre[q];
+    }
+  };
+  fetch(y0 - 1);
+  store(y0 - 1);
+  fetch(y0);
+  store(y0);
+  fetch(y0 + 1);
+  const int xo = 16 * w + lr;
+  for (int y = y0; y < y1; ++y) {
+    store(y + 1);
+    __syncthreads();
+    if (y + 1 < y1) fetch(y + 2);
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const int dy = kb / 3 - 1, dx = kb % 3 - 1;
+      acc = mfma16(a[kb], sRow[(y + dy + 3) % 3][(xo + 1 + dx) * CH + lg], acc);
+    }
+    const int x = x0 + xo;
+    if (lg == 0 && x < m.W) {
+      const float v = acc[0] + b4;
+      delta[(int64_t)y * m.W + x] = active(m, X0, y, x, gate) ? v : 0.f;
+    }
+    __syncthreads();
   }
 }
 
@@ -506,6 +575,11 @@ void launch_conv3x3(int cin, int mode, Img m, const __bf16* X, const __bf16* Wp,
     hipLaunchKernelGGL((k_conv3x3<NET_C, 1>), grid, dim3(256), 0, s, m, X, Wp, bias, mask, Y);
 }
 
+void launch_head_infer(Img m, const __bf16* X3, const __bf16* w4, const float* params, const __bf16* X0, float gate,
+                       float* delta, hipStream_t s) {
+  const dim3 grid((m.W + 63) / 64, (m.H + CONV_RB - 1) / CONV_RB);
+  hipLaunchKernelGGL(k_head_mfma, grid, dim3(256), 0, s, m, X3, w4, params, X0, gate, delta);
+}
 void launch_head_fwd(Img m, const __bf16* X3, const __bf16* w4, const float* params, const __bf16* X0, float gate,
                      float* delta, const float* target, float* G4, double* loss_part, int* nparts, hipStream_t s) {
   const int64_t npx = (int64_t)m.H * m.W;

@@ -110,7 +110,7 @@ int cdiff_net::forward(const __bf16* X0in, hipStream_t s) {
   launch_conv3x3(NET_C0, 0, m, X0in, w.w1p, params + NP_B1, nullptr, X1, s);
   launch_conv3x3(NET_C, 0, m, X1, w.w2p, params + NP_B2, nullptr, X2, s);
   launch_conv3x3(NET_C, 0, m, X2, w.w3p, params + NP_B3, nullptr, X3, s);
-  launch_head_fwd(m, X3, w.w4, params, X0in, gate, delta, nullptr, nullptr, nullptr, nullptr, s);
+  launch_head_infer(m, X3, w.w4, params, X0in, gate, delta, s);
   LAUNCH_CHECK("network forward");
   return CDIFF_OK;
 }
@@ -304,6 +304,22 @@ int cdiff_net_loss_grad(cdiff_net* n, int height, int width, const uint16_t* fea
   (void)hipFree(lsum);
   if (!rc && loss) *loss = l / (double)ni;
   return rc;
+}
+
+int cdiff_net_predict(cdiff_net* n, int height, int width, const uint16_t* features, float* delta) {
+  if (!n || !features || !delta || height < 1 || width < 1) return CDIFF_BAD_ARG;
+  const Img m{height, width};
+  int rc = n->workspace(m);
+  if (rc) return rc;
+  const int64_t ni = (int64_t)height * width;
+  hipError_t e = hipMemcpyAsync(n->X0, features, sizeof(uint16_t) * img_pixels(m) * NET_C0, hipMemcpyHostToDevice,
+                                n->stream);
+  if (e != hipSuccess) return n->hip_fail(e, "upload");
+  if ((rc = n->forward(n->X0, n->stream))) return rc;
+  e = hipMemcpyAsync(delta, n->delta, sizeof(float) * ni, hipMemcpyDeviceToHost, n->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+  if (e != hipSuccess) return n->hip_fail(e, "download");
+  return CDIFF_OK;
 }
 
 const char* cdiff_net_last_error(const cdiff_net* n) { return n ? n->err.c_str() : "null network"; }

@@ -370,10 +370,11 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   // learned warm start: features of (T^n, T^{n-1}) and the prediction, before
   // the prologue overwrites u_prev
   warm_pending = false;
+  warm_r0_ready = false;
   const bool warm_here = net && (o.precision == CDIFF_PREC_FP32_DC || o.precision == CDIFF_PREC_FP64) && ndim == 2 &&
                          lin_ok();
-  // after predictions lost to T^n, the network sits out 1, 3, 7 (at most 8)
-  // steps before it is consulted again
+  // after predictions that lost to T^n or won by less than half, the network
+  // sits out 1, 3, 7 (at most 8) steps before it is consulted again
   const bool warm_skipped = warm_here && warm_skip > 0;
   if (warm_skipped) --warm_skip;
   if (warm_here && !warm_skipped) {
@@ -386,6 +387,11 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
     if ((rc = learn_features(dt, net->X0, learn_ts))) return rc;
     if ((rc = net->forward(net->X0, stream))) return fail(rc, net->err);
     warm_pending = true;
+    // the feature pass evaluated the residual of T^n (u = u_prev = T^n, BC
+    // ghosts by the pass's own rule) with the 2D outer kernel; nothing before
+    // the step's first residual touches its partial sums unless the deeper
+    // Dirichlet halo layers are filled first
+    warm_r0_ready = lin && !dirichlet_outer;
   }
 
   // prologue: u = u_prev = T (PseudoTransientSolver.jl:229-230), validity (:238-250)
@@ -422,9 +428,15 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
         int rc = ensure_fp32();
         if (rc) return rc;
         dc_Vpdtau = Vpdtau;
-        if ((rc = dc_residual(dt, 0, nullptr, 1, 1, o))) return rc;
+        if ((rc = warm_r0_ready ? dc_finalize_outer(1, o) : dc_residual(dt, 0, nullptr, 1, 1, o))) return rc;
         HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        if (warm_r0_ready && ctrl_h[2].done) {
+          // T^n is already converged: its iterate with the BC ghosts in u[1]
+          if ((rc = dc_residual(dt, 0, nullptr, 1, 1, o))) return rc;
+          HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+          HIP_TRY(hipStreamSynchronize(stream));
+        }
         bool used = false;
         if (!ctrl_h[2].done && (rc = warm_start_apply(dt, o, &used))) return rc;
         st.warm_start = used ? 1 : 0;

@@ -120,6 +120,7 @@ struct cdiff_solver {
   double* learn_scratch = nullptr;  // >= 1024 doubles: reductions of the feature pass
   double* learn_ts = nullptr;       // device: [max |T^n|, dt max |r(T^n)|] of the current step
   bool warm_pending = false;        // a prediction for this step is in the network's workspace
+  bool warm_r0_ready = false;       // the feature pass left T^n's FP64 residual partial sums in place
   int warm_rejects = 0;             // consecutive steps whose prediction lost to T^n
   int warm_skip = 0;                // steps left before the network is consulted again
 
@@ -149,6 +150,7 @@ struct cdiff_solver {
   int bench_sweeps(double dt, int precision, int check, int n, double* mean_ms);
   int ensure_fp32();
   int dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o);
+  int dc_finalize_outer(int iter, const cdiff_opts& o);  // norm of the last outer pass's partial sums
   // the fused outer pass on explicit buffers (2D, lin_ok); finalize: update ctrl
   int dc_outer_pass(double dt, const double* u_old, const float* e, double* u_new, const double* uprev, int iter,
                     const cdiff_opts& o, bool finalize);

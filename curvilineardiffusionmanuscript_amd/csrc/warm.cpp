@@ -88,13 +88,23 @@ int cdiff_solver::warm_start_apply(double dt, const cdiff_opts& o, bool* used) {
   HIP_TRY(hipStreamSynchronize(stream));
   if (std::getenv("CDIFF_DC_DEBUG"))
     std::fprintf(stderr, "[dc] residual of the predicted start %.3e (of T^n %.3e)\n", ctrl_h[2].abs_, r0);
+  // the back-off: after a prediction that did not at least halve the
+  // residual, the network sits out 1, 3, 7 (at most 8) steps
+  auto back_off = [&]() {
+    ++warm_rejects;
+    warm_skip = std::min((1 << std::min(warm_rejects, 4)) - 1, 8);
+  };
   if (ctrl_h[2].abs_ < r0 && ctrl_h[2].status == CDIFF_OK) {
+    // the smaller residual wins; a marginal win does not pay for the next
+    // inference, so it counts as a loss for the back-off
     *used = true;
-    warm_rejects = 0;
+    if (ctrl_h[2].abs_ <= 0.5 * r0)
+      warm_rejects = 0;
+    else
+      back_off();
     return CDIFF_OK;
   }
-  ++warm_rejects;
-  warm_skip = std::min((1 << std::min(warm_rejects, 4)) - 1, 8);
+  back_off();
   // fallback: start from T^n (in u_prev) again
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, o.rtol, o.atol);
   hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)u_prev, u[0]);

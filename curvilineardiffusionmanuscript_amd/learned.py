@@ -140,6 +140,15 @@ class WarmStartNet:
                                                  C.byref(loss)))
         return loss.value, g, d.reshape(H, W)
 
+    def predict(self, features_bf16: np.ndarray, height: int, width: int) -> np.ndarray:
+        """Prediction for host features ((H+2, W+2, 8) bf16 bit patterns) through
+        the inference kernels the warm start runs."""
+        f = np.ascontiguousarray(features_bf16, dtype=np.uint16)
+        d = np.zeros(height * width, dtype=np.float32)
+        self._check(L.load().cdiff_net_predict(self._h, int(height), int(width), f.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                               d.ctypes.data_as(C.POINTER(C.c_float))))
+        return d.reshape(height, width)
+
     # -- use --------------------------------------------------------------------
     def attach(self, solver: S.PseudoTransientSolver):
         rc = L.load().cdiff_set_warm_start(solver._h, self._h)

@@ -282,6 +282,10 @@ int cdiff_net_eval(cdiff_net *net, int slot, float *delta, double *loss);
  * NULL).  Test hook for the backward kernels. */
 int cdiff_net_loss_grad(cdiff_net *net, int height, int width, const uint16_t *features, const float *target,
                         float *grad, float *delta, double *loss);
+/* Prediction (height*width floats, row-major by j) for host-provided
+ * features, through the inference kernels the warm start uses (MFMA head);
+ * no counterpart in the reference, whose solver has no learned part. */
+int cdiff_net_predict(cdiff_net *net, int height, int width, const uint16_t *features, float *delta);
 const char *cdiff_net_last_error(const cdiff_net *net);
 /* Activity threshold: the prediction is 0 (and carries no gradient) on cells
  * whose 3x3 neighbourhood has |dt r(T^n)| <= gate * dt max|r(T^n)|

@@ -47,6 +47,29 @@ def test_loss_gradient_matches_oracle(H, W):
     net.close()
 
 
+@pytest.mark.parametrize("H,W", [(37, 70), (64, 64)])
+def test_inference_prediction_matches_oracle_and_training_head(H, W):
+    # the warm start's inference kernels (MFMA head) against the oracle's
+    # prediction, and against the training path's prediction (same conv layers,
+    # vector-ALU head): those two differ only in the FP32 summation order
+    p = N.init_params(21)
+    X = N.random_features(H, W, 22)
+    X[H // 2 :, : W // 4 + 1, 5] = 0.0  # an inactive region: prediction 0 there
+    Xb = N.bf16_round(X)
+    net = LW.WarmStartNet()
+    net.params = p
+    pred = net.predict(N.bf16_bits(X), H, W)
+    _, _, d_train = net.loss_grad(N.bf16_bits(X), np.zeros((H, W), np.float32))
+    _, _, do = N.loss_grad(p.astype(np.float64), Xb.astype(np.float64), np.zeros((H, W)), bf16=True)
+    err_o = np.linalg.norm(pred - do) / np.linalg.norm(do)
+    err_t = np.max(np.abs(pred - d_train)) / np.max(np.abs(d_train))
+    print(f"{H}x{W}: |pred - oracle| {err_o:.2e}, max|pred - training head| {err_t:.2e}")
+    assert err_o <= 1e-2
+    assert err_t <= 1e-5
+    assert np.all(pred[d_train == 0.0] == 0.0)
+    net.close()
+
+
 def test_training_on_captured_pairs_reduces_loss():
     n, full = 48, 512
     net = LW.WarmStartNet(seed=1)
