@@ -23,6 +23,7 @@ struct LinOut {
   float* fy[3];
   float* fz[3];
   float* cc;
+  double dtau_scale, theta_scale;  // of dtau and theta (1: the reference's)
 };
 __global__ void k_lin_metrics3d(Geo g, DcCoefs cf, float* m0, float* m1, float* m2, float* m3, float* m4, float* m5,
                                 float* m6, float* m7, float* m8);
@@ -36,6 +37,7 @@ struct DcOuter2D {
   IterConst ic;
   int harmonic, newton, ntiles, chunks;
   double t_exp;
+  double dtau_scale, theta_scale;  // of the correction sweep's pseudo-time step and damping (1: the reference's)
   const double* rho;
   const double* u_old;
   const float* e;  // nullable: no correction
@@ -61,6 +63,7 @@ struct DcOuter3D {
   IterConst ic;
   int harmonic, newton, ntiles, chunks, stripsI, ch;
   double t_exp;
+  double dtau_scale, theta_scale;  // of the correction sweep's pseudo-time step and damping (1: the reference's)
   const double* rho;
   const double* u_old;
   const float* e;  // nullable: no correction

@@ -309,6 +309,8 @@ __global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double 
   }
   auto params = [&](int64_t cell, double& dtau, double& theta) {
     iter_params<double>(ic, rho ? rho[cell] : 1.0, a[cell], dtau, theta);
+    dtau *= out.dtau_scale;
+    theta *= out.theta_scale;
   };
   double dtau0 = 0.0, th0 = 0.0;
   if (near) params(c, dtau0, th0);

@@ -96,6 +96,8 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
       a = alpha_of<double>(A.kap, rho, u);
       double dtau;
       iter_params<double>(A.ic, rho, a, dtau, th);
+      dtau *= A.dtau_scale;
+      th *= A.theta_scale;
       da = dadu(A.t_exp, a, u);
       const bool own_r = cy >= 1 && cy <= OTY && r <= g.hi[1];
       const bool own_i = cx >= 1 && cx <= OTX && i <= g.hi[0];

@@ -148,6 +148,8 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
         a = alpha_of<double>(A.kap, C[s].rho, u);
         double dtau;
         iter_params<double>(A.ic, C[s].rho, a, dtau, th);
+        dtau *= A.dtau_scale;
+        th *= A.theta_scale;
         da = dadu3(A.t_exp, a, u);
         const int i = gx + rlx[s], j = gy + rly[s];
         const bool gi = i < g.lo[0] || i > g.hi[0], gj = j < g.lo[1] || j > g.hi[1];

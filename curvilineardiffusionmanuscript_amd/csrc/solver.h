@@ -81,6 +81,11 @@ struct cdiff_solver {
   double* lin_pmax = nullptr;
   int lin_nparts = 0;
   double dc_Vpdtau = 0.0;
+  // extra scales of the correction sweep's dtau and theta (the solver's
+  // defaults, negative until the first mixed-precision step reads them; 1
+  // after the sweeps diverged once)
+  double dc_dtau_scale = 1.0, dc_theta_scale = 1.0;
+  double dc_dtau_scale_dflt = -1.0, dc_theta_scale_dflt = -1.0;
   double dc_cfl_scale = -1.0;  // pseudo-time step scale of the correction sweeps (< 0: not chosen yet)
 
   // sweep decomposition
