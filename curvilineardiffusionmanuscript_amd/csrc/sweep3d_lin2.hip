@@ -16,8 +16,10 @@
 // The coefficients are read once for both iterations and e, q once in and
 // once out; every value is computed by the same FP32 expressions, in the same
 // order, as two launches of the one-iteration sweep, so the results are
-// bit-identical to those.  Loads of plane t+1 and of the update coefficients
-// of plane t are in flight while steps 3-5 run.
+// bit-identical to those.  Loads of the next plane (two planes without the
+// convergence check) and of the update coefficients of plane t are in flight
+// while steps 3-5 run; the face coefficients of a plane wait for their second
+// use one step later in the thread's own LDS slots.
 #include "cdiff_dev.h"
 #include "sweep_args.h"
 
@@ -38,6 +40,7 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
   __shared__ float sE1[2][Q2], sX2[3][Q2], sY2[3][Q2], sZ2[2][Q2];
   __shared__ float sPX[CHECK ? 3 : 1][CHECK ? Q2 : 1], sPY[CHECK ? 3 : 1][CHECK ? Q2 : 1];
   __shared__ float sPZ[CHECK ? 2 : 1][CHECK ? Q2 : 1];
+  __shared__ float sC[2][9][Q2];  // per-thread store of the face coefficients between plane steps
   __shared__ double sRed[2][T2N / WAVE];
   const Geo& g = A.g;
   const int tid = threadIdx.x;
@@ -104,10 +107,6 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
       F.f[2][3] = A.zero_src ? 0.f : A.q_src[2][cz];
     }
   };
-  // face coefficients kept for the iteration-2 faces one step later
-  struct Coef {
-    float f[3][3];
-  };
   struct Upd {
     float rb, cc, m[9];
   };
@@ -134,12 +133,18 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
            U.m[6] * (x_eta + y_xi) + U.m[7] * (x_zeta + z_xi) + U.m[8] * (y_zeta + z_eta);
   };
 
-  Fetch F;
-  Coef C{}, Cp{};
+  // loads of the next planes: step t consumes the register set of plane t and
+  // refills it with plane t+PD.  Without the check, two sets (PD = 2, the loop
+  // unrolled by two to keep them in registers) put two planes' loads in flight
+  // behind the update; the check variant's residual needs the registers of the
+  // second set (PD = 1)
+  constexpr int PD = CHECK ? 1 : 2;
+  Fetch FA, FB;
   Upd U1{}, U2{}, Un{};
-  fetch(z0 - 2, F, false);
+  fetch(z0 - 2, FA, false);
+  if (PD == 2) fetch(z0 - 1, FB, true);
   double sq = 0.0, smax = 0.0;
-  for (int t = z0 - 2; t <= z1 + 2; ++t) {
+  auto step = [&](int t, Fetch& F) {
     // ---- 1-2: e0(t) into LDS; iteration-1 faces of plane t (+ zeta faces t-1|t)
     if (valid) sE0[t & 1][r] = F.e;
     __syncthreads();
@@ -162,12 +167,16 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
         sZ1[(t + 1) & 1][r] = pz + F.f[2][2] * (F.f[2][3] - pz);
       }
     }
+    // the face coefficients of plane t, for its iteration-2 faces one step
+    // later, in this thread's own LDS slots (no barrier needed)
+    if (inner) {
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax)
+      for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) C.f[ax][q] = F.f[ax][q];
+        for (int q = 0; q < 3; ++q) sC[t & 1][3 * ax + q][r] = F.f[ax][q];
+    }
     // loads of the next plane step and of the update coefficients of plane t
-    if (t < z1 + 2) fetch(t + 1, F, t + 1 >= z0 - 1);
+    if (t + PD <= z1 + 2) fetch(t + PD, F, t + PD >= z0 - 1);  // F is free now
     if (t >= z0 - 1 && t <= z1 + 1) fetch_upd(t, Un);
     __syncthreads();
 
@@ -189,24 +198,25 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
 
       // ---- 4: iteration-2 faces of plane t-1 (+ zeta faces t-2|t-1)
       if (inner) {
+        const float(*Cf)[Q2] = sC[p1 & 1];
         const float* E = sE1[p1 & 1];
         const float* Em = sE1[(p1 + 1) & 1];
         const float ec = E[r];
         const int s = (p1 + 3) % 3;
         if (xf2) {
-          const float px = Cp.f[0][0] * ec - Cp.f[0][1] * E[r + 1];
-          sX2[s][r] = px + Cp.f[0][2] * (sX1[s][r] - px);
+          const float px = Cf[0][r] * ec - Cf[1][r] * E[r + 1];
+          sX2[s][r] = px + Cf[2][r] * (sX1[s][r] - px);
           if (CHECK) sPX[s][r] = px;
         }
         if (yf2) {
-          const float py = Cp.f[1][0] * ec - Cp.f[1][1] * E[r + Q2X];
-          sY2[s][r] = py + Cp.f[1][2] * (sY1[s][r] - py);
+          const float py = Cf[3][r] * ec - Cf[4][r] * E[r + Q2X];
+          sY2[s][r] = py + Cf[5][r] * (sY1[s][r] - py);
           if (CHECK) sPY[s][r] = py;
         }
         if (p1 >= z0) {
           const int z = (p1 + 1) & 1;  // face p1-1 | p1
-          const float pz = Cp.f[2][0] * Em[r] - Cp.f[2][1] * ec;
-          sZ2[z][r] = pz + Cp.f[2][2] * (sZ1[z][r] - pz);
+          const float pz = Cf[6][r] * Em[r] - Cf[7][r] * ec;
+          sZ2[z][r] = pz + Cf[8][r] * (sZ1[z][r] - pz);
           if (CHECK) sPZ[z][r] = pz;
         }
       }
@@ -237,9 +247,16 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
         if (p2 == g.lo[2]) A.q_dst[2][c - g.plane] = sZ2[zm][r];
       }
     }
-    Cp = C;
     U2 = U1;
     U1 = Un;
+  };
+  if constexpr (PD == 1) {
+    for (int t = z0 - 2; t <= z1 + 2; ++t) step(t, FA);
+  } else {
+    for (int t = z0 - 2; t <= z1 + 2; t += 2) {
+      step(t, FA);
+      if (t + 1 <= z1 + 2) step(t + 1, FB);
+    }
   }
   if (CHECK) {
     sq = wave_sum(sq);
