@@ -105,6 +105,28 @@ def test_config2_hot_disk_nonlinear_dirichlet_adaptive():
     _assert_parity(out, err_rel=1e-3)
 
 
+def test_config1_full_size():
+    # the CPU-runnable configuration as stated: 128 x 128 cells, wavy map
+    # A = 0.03, kappa = 1, T0 = 1 + 10 exp(-r^2/0.01), zero flux, fixed dt =
+    # 1e-4, 10 steps, tolerance 1e-8
+    mesh, T0 = gaussian_2d(128)
+    _, _, out = _run_pair(mesh, T0, ["neumann"] * 4, (1.0, 0.0, 0.0), 10, 1e-4, rtol=0.0, atol=1e-8,
+                          max_iter=10**6, calculate_next_dt=False)
+    _assert_parity(out, rel=0.0)
+
+
+def test_config2_full_size():
+    # the drop-in configuration as stated: 1024 x 1024 cells, wavy map A =
+    # 0.05, kappa = T^2.5, background 0.01 with a seeded hot disk T = 1 of
+    # radius 0.05, Dirichlet T = 1 on the left edge, zero flux elsewhere,
+    # adaptive dt (dt0 = 1e-5), 50 steps, tolerance 1e-8 (the oracle takes
+    # ~15 s with 16 threads)
+    mesh, T0 = hot_disk_2d(1024, seed=0)
+    _, _, out = _run_pair(mesh, T0, [("dirichlet", 1.0), "neumann", "neumann", "neumann"], (1.0, 2.5, 0.0), 50,
+                          1e-5, adaptive=True, rtol=0.0, atol=1e-8, max_iter=10**6)
+    _assert_parity(out, err_rel=1e-3)
+
+
 @pytest.mark.parametrize(
     "bcs",
     [
