@@ -100,7 +100,12 @@ int cdiff_solver::ensure_fp32() {
     owned.push_back(ctrl_in);
   }
   const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  const int nres = (int)((nfull + 255) / 256);
+  // one partial per 256 cells (per-cell residual kernels) or per tile of the
+  // fused outer pass, whose block count is rounded up to a multiple of 8
+  // (64 x 8 tiles; in 3D at most one plane per chunk)
+  const int64_t tiles = (int64_t)((g.hi[0] - g.lo[0] + 64) / 64) * ((g.hi[1] - g.lo[1] + 8) / 8) *
+                        (ndim == 3 ? (g.hi[2] - g.lo[2] + 1) : 1);
+  const int nres = (int)std::max<int64_t>((nfull + 255) / 256, (tiles + 7) / 8 * 8);
   if (nres > dc_nparts) {
     HIP_TRY(hipMalloc((void**)&dc_psum, sizeof(double) * nres));
     owned.push_back(dc_psum);

@@ -125,6 +125,29 @@ def test_fused_3d_outer_pass_matches_separate_kernels(ch, monkeypatch):
     np.testing.assert_allclose(out["1"][0], out["0"][0], rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("ni,nj,neg", [(70, 45, True), (129, 17, False), (9, 5, False)])
+def test_dc_small_ragged_meshes(ni, nj, neg):
+    # small meshes whose outer-pass tiles (64 x 8, block count rounded up to a
+    # multiple of 8) outnumber their 256-cell groups: a hot disk, Dirichlet,
+    # zero-flux and mirror faces, two steps to the FP64 tolerance
+    from curvilineardiffusionmanuscript_amd.mesh import CurvilinearGrid2D, wavy_map_2d
+
+    mesh = CurvilinearGrid2D.from_mapping(wavy_map_2d(0.05), (ni, nj), 1)
+    xc, yc = mesh.centroid_coordinates.x, mesh.centroid_coordinates.y
+    T0 = np.asfortranarray(np.where((xc - 0.45) ** 2 + (yc - 0.55) ** 2 <= 0.15**2, 1.0, 0.01))
+    bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.FixedNegSymmetryBC() if neg else S.NeumannBC(), S.NeumannBC()]
+    law = S.PowerLawConductivity(1.0, 2.5, 0.0)
+    dev = S.PseudoTransientSolver(mesh, bcs)
+    T = T0.copy(order="F")
+    S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
+    for _ in range(2):
+        st, _ = S.nonlinear_thermal_conduction_step(dev, mesh, T, 1.0, 1.0, law, 1e-5, rtol=0.0, atol=1e-8,
+                                                    max_iter=10**6, precision="fp32_dc", calculate_next_dt=False,
+                                                    show_convergence=False)
+        assert st.abs_err <= 1e-8 and st.niter > 0
+    dev.close()
+
+
 @pytest.mark.parametrize("ch,chk", [("11", "2"), ("4", "3"), ("1", "4"), ("3", "5")])
 def test_lin3_two_iteration_launch_matches_one(ch, chk, monkeypatch):
     # the 3D correction sweep with two iterations per launch (two-cell ring,
