@@ -101,6 +101,7 @@ class Stats(C.Structure):
         ("time", C.c_double),
         ("outer", C.c_int32),
         ("warm_start", C.c_int32),
+        ("fp64_fallback", C.c_int32),
     ]
 
 

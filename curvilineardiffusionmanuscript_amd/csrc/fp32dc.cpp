@@ -585,9 +585,19 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   int64_t inner_total = 0;
   int inner_checks = 0;
   const int64_t nq = g.total;
-  while (!ctrl_h[2].done) {
+  // the defect correction is abandoned (and the step redone by the FP64 APT)
+  // when its outer residual turns non-finite, grows past 100x the first one
+  // or has not converged after max_outer rounds
+  dc_failed = false;
+  const double r_first = ctrl_h[2].abs_;
+  auto failed_now = [&]() {
+    const Ctrl& c = ctrl_h[2];
+    return c.status == CDIFF_NONFINITE || !(c.abs_ == c.abs_) || c.abs_ > 100.0 * r_first;
+  };
+  if (failed_now()) dc_failed = true;
+  while (!ctrl_h[2].done && !dc_failed) {
     if (outer >= max_outer) {
-      ctrl_h[2].status = CDIFF_MAX_ITER;
+      dc_failed = true;
       break;
     }
     // inner FP32 solve for the correction, from e = 0 and q = 0 (the
@@ -677,6 +687,10 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     if (dc_debug)
       std::fprintf(stderr, "[dc] outer %d: inner %d (inner residual %.3e), outer residual %.3e\n", outer, ni,
                    ctrl_h[3].last_l2, ctrl_h[2].abs_);
+    if (failed_now()) {
+      dc_failed = true;
+      break;
+    }
     if (inner_total > total_cap && !ctrl_h[2].done) {
       ctrl_h[2].status = CDIFF_MAX_ITER;
       break;

@@ -414,10 +414,29 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
       LAUNCH_CHECK("k_halo_fill");
     }
 
+  bool fallback = false;
   if (o.precision == CDIFF_PREC_FP32_DC) {
     int rc = step_fp32_dc(dt, o, Vpdtau, &st);
     if (rc) return rc;
-  } else if (o.precision == CDIFF_PREC_FP64) {
+    if (dc_failed) {
+      // the defect correction gave up: redo the step with the FP64 APT from
+      // T^n (Tdev, u_prev and the FP64 fluxes are untouched by the correction
+      // loop), so the step ends like an FP64 step would
+      if (std::getenv("CDIFF_DC_DEBUG"))
+        std::fprintf(stderr, "[dc] defect correction gave up after %d rounds; FP64 APT from T^n\n", st.outer);
+      fallback = true;
+      for (double* d : {u[0], u[1], u_prev}) {
+        hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)Tdev,
+                           d);
+        LAUNCH_CHECK("k_copy");
+      }
+      reset_and_validate();
+      if (dirichlet_outer)
+        for (int b = 0; b < 2; ++b) hipLaunchKernelGGL(k_halo_fill, grid_full(g), dim3(256), 0, stream, g, gh, u[b], 1);
+      LAUNCH_CHECK("fallback prologue");
+    }
+  }
+  if (o.precision == CDIFF_PREC_FP64 || fallback) {
     if (warm_pending) {
       // learned warm start of the FP64 APT (valid inputs only): the start with
       // the smaller FP64 residual, prediction or T^n, with its BC ghosts, in
@@ -466,12 +485,13 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
     }
     int rc = run_loop64(o, dt, Vpdtau, 0);
     if (rc) return rc;
-  } else {
+  } else if (o.precision != CDIFF_PREC_FP32_DC) {
     return fail(CDIFF_BAD_ARG, "unknown precision mode");
   }
 
   const Ctrl& c = ctrl_h[2];
-  const bool dc = o.precision == CDIFF_PREC_FP32_DC;
+  const bool dc = o.precision == CDIFF_PREC_FP32_DC && !fallback;
+  st.fp64_fallback = fallback ? 1 : 0;
   if (!dc) {
     st.rel_err = c.rel;
     st.abs_err = c.abs_;

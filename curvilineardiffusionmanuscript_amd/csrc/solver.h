@@ -125,6 +125,7 @@ struct cdiff_solver {
   double* learn_scratch = nullptr;  // >= 1024 doubles: reductions of the feature pass
   double* learn_ts = nullptr;       // device: [max |T^n|, dt max |r(T^n)|] of the current step
   bool warm_pending = false;        // a prediction for this step is in the network's workspace
+  bool dc_failed = false;  // the last mixed-precision step gave up (diverged or did not converge)
   bool warm_r0_ready = false;       // the feature pass left T^n's FP64 residual partial sums in place
   int warm_rejects = 0;             // consecutive steps whose prediction lost to T^n
   int warm_skip = 0;                // steps left before the network is consulted again

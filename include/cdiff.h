@@ -124,6 +124,9 @@ typedef struct {
   double time;     /* seconds, device-synchronised wall time of the step */
   int32_t outer;   /* FP32_DC: FP64 defect-correction rounds */
   int32_t warm_start; /* 1 if the step started from the learned prediction */
+  int32_t fp64_fallback; /* FP32_DC: 1 if the defect correction gave up (diverged or did not converge)
+                          * and the step was redone by the FP64 APT from T^n; the other fields are
+                          * then the FP64 APT's */
 } cdiff_stats;
 
 /* Fields for cdiff_get_field / cdiff_set_field. */
