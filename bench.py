@@ -12,13 +12,14 @@ solve to tolerance.  The temperature stays in HBM for the whole timed region.
 x = xi + 0.04 sin(2 pi xi) sin(2 pi eta) sin(2 pi zeta) (same for y, z),
 kappa = T^3, 8 seeded Gaussian spots on T = 0.01, zero-flux faces, dt0 = 1e-5.
 
-Timed in the same run: the headline method (FP32 correction sweeps inside
-the FP64 defect-correction loop) and the plain FP64 APT, from the same
-state; `value` is the headline method's s/time-step.  In 2D the same
-mixed-precision steps also run with the learned warm start attached (the
-committed network; config.learned_warm_start): each step starts from the
-network's prediction when that has the smaller FP64 residual, and `value`
-stays the step without the network, the faster of the two on this workload.
+Timed in the same run, from the same state: the mixed-precision step (FP32
+correction sweeps inside the FP64 defect-correction loop) with the learned
+warm start attached (the committed network: each step starts from the
+network's prediction when that has the smaller FP64 residual, else from
+T^n), the same step without the network (config.without_learned_warm_start)
+and the plain FP64 APT.  `value` is the s/time-step of the first, the
+method the metric names (FP32 + learned); in 3D, where the warm start does
+not apply, of the mixed-precision step.
 
 Prints one JSON line: s/time-step (lower is better), with
   roofline      the dominant kernel's algorithmic bytes per launch (cells x
@@ -126,7 +127,7 @@ def run_gpu(args, mesh, T0, method, net=None):
 # one launch runs and the cdiff_bench_sweeps mode bits that launch it
 KERNELS = {
     ("fp64", 2): ("sweep2d_pair_kernel<double, false, 8>", 2, 0 | 2),
-    ("fp32_dc", 2): ("sweep2d_lin_kernel<4, 16, false>", 4, 4),
+    ("fp32_dc", 2): ("sweep2d_lin_rows_kernel<6, 8, 4, false>", 6, 4),
     ("fp64", 3): ("sweep3d_col_kernel<double, false, true>", 1, 0),
     ("fp32_dc", 3): ("sweep3d_lin2_kernel<false>", 2, 4),
 }
@@ -139,7 +140,7 @@ def sol_bench(sol, dt, method, n):
 
     out = C.c_double()
     prec = L.PREC_FP64 if method == "fp64" else L.PREC_FP32_DC
-    n = n - n % 4
+    n = max(24, n - n % 24)  # a multiple of every depth the sweeps can run per launch
     rc = L.load().cdiff_bench_sweeps(sol._h, float(dt), prec, KERNELS[(method, sol.ndim)][2], int(n), C.byref(out))
     if rc:
         raise RuntimeError(L.load().cdiff_last_error(sol._h).decode())
@@ -232,38 +233,44 @@ def main():
         net.close()
     n_cells = args.n**ndim
     bpc = BYTES_PER_CELL[(args.method, ndim)]
-    s_per_step = g["elapsed"] / args.steps
-    it_mean = float(np.mean(g["iters"]))
-    teff = bpc * n_cells * sum(g["iters"]) / g["elapsed"] / 1e9
-    log(f"GPU {args.method}: {s_per_step * 1e3:.3f} ms/step, {it_mean:.1f} its/step, T_eff(step) {teff:.1f} GB/s, "
+    # the headline run: the mixed-precision step with the learned warm start
+    # where it applies (2D), else the method itself
+    head = lrn if lrn is not None else g
+    s_per_step = head["elapsed"] / args.steps
+    it_mean = float(np.mean(head["iters"]))
+    teff = bpc * n_cells * sum(head["iters"]) / head["elapsed"] / 1e9
+    s_plain = g["elapsed"] / args.steps
+    log(f"GPU {args.method}: {s_plain * 1e3:.3f} ms/step, {np.mean(g['iters']):.1f} its/step, "
         f"sweep {g['sweep_ms'] * 1e3:.1f} us/iteration")
+    if lrn is not None:
+        log(f"GPU {args.method}+learned: {s_per_step * 1e3:.3f} ms/step, {it_mean:.1f} its/step, warm start "
+            f"used on {lrn['warm_used']} of {args.steps} steps, T_eff(step) {teff:.1f} GB/s")
     extra = {}
+    if lrn is not None:
+        extra["warm_start_used_steps"] = lrn["warm_used"]
+        extra["without_learned_warm_start"] = {
+            "s_per_step": round(s_plain, 6),
+            "iters_per_step": round(float(np.mean(g["iters"])), 2),
+            "outer_per_step": round(float(np.mean(g["outer"])), 2),
+            "max_abs_err": max(g["abs_err"]),
+        }
     if ref is not None:
         s64 = ref["elapsed"] / args.steps
         scale = float(np.max(np.abs(ref["T_end"])))
-        err = float(np.max(np.abs(g["T_end"] - ref["T_end"]))) / scale
+        err = float(np.max(np.abs(head["T_end"] - ref["T_end"]))) / scale
         log(f"GPU fp64: {s64 * 1e3:.3f} ms/step, {np.mean(ref['iters']):.1f} its/step; speedup "
-            f"{s64 / s_per_step:.2f}x; max|T - T_fp64|/max|T| = {err:.2e}")
-        extra = {
+            f"{s64 / s_per_step:.2f}x (without the network {s64 / s_plain:.2f}x); "
+            f"max|T - T_fp64|/max|T| = {err:.2e}")
+        extra.update({
             "fp64_s_per_step": round(s64, 6),
             "fp64_iters_per_step": round(float(np.mean(ref["iters"])), 2),
             "speedup_vs_fp64": round(s64 / s_per_step, 3),
             "max_rel_diff_vs_fp64": err,
-        }
-    if lrn is not None:
-        sl = lrn["elapsed"] / args.steps
-        log(f"GPU {args.method}+learned: {sl * 1e3:.3f} ms/step, {np.mean(lrn['iters']):.1f} its/step, warm start "
-            f"used on {lrn['warm_used']} of {args.steps} steps")
-        extra["learned_warm_start"] = {
-            "s_per_step": round(sl, 6),
-            "iters_per_step": round(float(np.mean(lrn["iters"])), 2),
-            "warm_start_used_steps": lrn["warm_used"],
-            "max_abs_err": max(lrn["abs_err"]),
-        }
-        if ref is not None:
-            extra["learned_warm_start"]["speedup_vs_fp64"] = round(s64 / sl, 3)
-            extra["learned_warm_start"]["max_rel_diff_vs_fp64"] = float(
-                np.max(np.abs(lrn["T_end"] - ref["T_end"]))) / scale
+        })
+        if lrn is not None:
+            w = extra["without_learned_warm_start"]
+            w["speedup_vs_fp64"] = round(s64 / s_plain, 3)
+            w["max_rel_diff_vs_fp64"] = float(np.max(np.abs(g["T_end"] - ref["T_end"]))) / scale
 
     kname, per_launch, _ = KERNELS[(args.method, ndim)]
     per_it_ms = g["sweep_ms"]
@@ -298,7 +305,7 @@ def main():
             f"same inputs and tolerance, FP64, oracle/apt_oracle.c with OpenMP",
         }
     out = {
-        "metric": "s/time-step + T_eff %HBM peak: FP32+defect-correction vs FP64 APT, same 1e-8 residual",
+        "metric": "s/time-step + T_eff %HBM peak: FP32+learned vs FP64 APT, same 1e-8 residual",
         "value": round(s_per_step, 6),
         "unit": "s/time-step",
         "n_gpus": 1,
@@ -315,14 +322,14 @@ def main():
             "workload": (
                 f"2D {args.n}x{args.n} wavy grid (A=0.05), kappa=T^2.5, hot disk seed {args.seed}, "
                 f"Dirichlet T=1 left, zero-flux elsewhere, adaptive dt0={args.dt0:g}, atol={args.tol:g}, "
-                f"method={args.method}"
+                f"method={args.method}{'+learned_warm_start' if lrn is not None else ''}"
                 if ndim == 2
                 else f"3D {args.n}^3 wavy grid (A=0.04), kappa=T^3, 8 Gaussian spots seed {args.seed}, "
                 f"zero-flux, adaptive dt0={args.dt0:g}, atol={args.tol:g}, method={args.method}"
             ),
             "iters_per_step": round(it_mean, 2),
-            "outer_per_step": round(float(np.mean(g["outer"])), 2),
-            "max_abs_err": max(g["abs_err"]),
+            "outer_per_step": round(float(np.mean(head["outer"])), 2),
+            "max_abs_err": max(head["abs_err"]),
             "t_eff_step_GBs": round(teff, 1),
             "t_eff_step_frac": round(teff / HBM_PEAK_GBS, 4),
             **extra,

@@ -25,7 +25,10 @@
 // on W and C).  A sweep then costs ~40 FP32 operations per cell and the
 // kernel is bound by memory, so each launch runs up to K iterations on a
 // tile with a K-cell ring held in LDS (temporal blocking): the coefficients
-// are read once per launch, and e, q once in and once out.
+// are read once per launch, and e, q once in and once out.  Two thread
+// layouts: 64-column tiles with the region cells spread over the threads
+// (sweep2d_lin_kernel), and wave-aligned region rows with tiles of 64 - 2K
+// columns (sweep2d_lin_rows_kernel, the default).
 #include "cdiff_dev.h"
 #include "sweep_args.h"
 
@@ -204,6 +207,176 @@ __global__ __launch_bounds__(LNTHR) void sweep2d_lin_kernel(Lin2DArgs A) {
   }
 }
 
+// The same sweep with region rows aligned to wavefronts: the region is 64
+// columns wide (a tile of 64 - 2K columns plus the K-cell ring on each
+// side), so lane l of every wave holds region column l and wave w holds the
+// region rows w, w + NW, w + 2 NW, ...  Row bounds are then wave-uniform
+// (scalar branches) and no per-slot index decoding is needed.  A block of NW
+// waves owns a tile of (64 - 2K) x (NW NSR - 2K) cells.  The arithmetic of
+// every face and cell is the one of sweep2d_lin_kernel, in the same order.
+template <int K, int NW, int NSR, bool CHECK>
+__global__ __launch_bounds__(NW * WAVE) void sweep2d_lin_rows_kernel(Lin2DArgs A) {
+  constexpr int RX = WAVE, RY = NW * NSR, NR = RX * RY;
+  constexpr int TX = RX - 2 * K, TY = RY - 2 * K;
+  if (A.ctrl->done) return;
+  __shared__ float sE[NR], sX[NR], sY[NR];
+  __shared__ float sPX[CHECK ? NR : 1], sPY[CHECK ? NR : 1];
+  __shared__ double sRed[2][NW];
+
+  const Geo& g = A.g;
+  const int tid = threadIdx.x;
+  const int lx = tid & (WAVE - 1);
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nb = gridDim.x, per = nb >> 3, b = blockIdx.x;
+  const int tile = (b & 7) * per + (b >> 3);
+  if (tile >= A.ntiles) {
+    if (CHECK && tid == 0) {
+      A.part_sum[b] = 0.0;
+      A.part_max[b] = 0.0;
+    }
+    return;
+  }
+  const int bx = tile / A.chunks, by = tile - bx * A.chunks;
+  const int x0 = g.lo[0] + bx * TX, y0 = g.lo[1] + by * TY;
+  const int i = x0 - K + lx;  // global column of this lane
+  const int gy = y0 - K;      // global row of region row 0
+  const float inv_dt = A.inv_dt;
+  const bool icol = i >= g.lo[0] && i <= g.hi[0];
+  const bool xcol = i >= g.lo[0] - 1 && i <= g.hi[0];
+
+  float e[NSR], qx[NSR], qy[NSR], rb[NSR], cc[NSR], m0[NSR], m1[NSR], m2[NSR], m3[NSR], m4[NSR];
+  float px[NSR], mx[NSR], wx[NSR], py[NSR], my[NSR], wy[NSR];
+  unsigned dom = 0;  // bit p: region cell of slot p is a domain cell
+#pragma unroll
+  for (int p = 0; p < NSR; ++p) {
+    const int ly = w + NW * p, r = gy + ly;
+    e[p] = qx[p] = qy[p] = rb[p] = cc[p] = m0[p] = m1[p] = m2[p] = m3[p] = m4[p] = 0.f;
+    px[p] = mx[p] = wx[p] = py[p] = my[p] = wy[p] = 0.f;
+    const bool rin = r >= g.lo[1] && r <= g.hi[1];
+    const bool d = icol && rin;
+    const bool xon = xcol && rin;
+    const bool yon = icol && r >= g.lo[1] - 1 && r <= g.hi[1];
+    const int64_t c = at(g, i, r, 0);
+    if (d) {
+      dom |= 1u << p;
+      if (!A.zero_src) e[p] = A.e_src[c];
+      rb[p] = A.base[c];
+      cc[p] = A.cc[c];
+      m0[p] = A.m[0][c];
+      m1[p] = A.m[1][c];
+      m2[p] = A.m[2][c];
+      m3[p] = A.m[3][c];
+      m4[p] = A.m[4][c];
+    }
+    if (xon) {
+      if (!A.zero_src) qx[p] = A.qx_src[c];
+      px[p] = A.fx[0][c];
+      mx[p] = A.fx[1][c];
+      wx[p] = A.fx[2][c];
+    }
+    if (yon) {
+      if (!A.zero_src) qy[p] = A.qy_src[c];
+      py[p] = A.fy[0][c];
+      my[p] = A.fy[1][c];
+      wy[p] = A.fy[2][c];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NSR; ++p) {
+    const int t = (w + NW * p) * RX + lx;
+    sE[t] = e[p];
+    sX[t] = qx[p];
+    sY[t] = qy[p];
+  }
+  __syncthreads();
+
+  double sq = 0.0, smax = 0.0;
+  for (int s = K - A.nit; s < K; ++s) {
+    const bool last = s == K - 1;
+    const bool fx_on = lx >= s && lx <= RX - 2 - s;
+    const bool fy_on = lx >= s && lx <= RX - 1 - s;
+    const bool c_on = lx >= s + 1 && lx < RX - 1 - s;
+#pragma unroll
+    for (int p = 0; p < NSR; ++p) {
+      const int ly = w + NW * p;
+      const int t = ly * RX + lx;
+      if (ly >= s && ly <= RY - 1 - s) {
+        if (fx_on) {
+          const float qp = px[p] * sE[t] - mx[p] * sE[t + 1];
+          qx[p] = qp + wx[p] * (qx[p] - qp);
+          sX[t] = qx[p];
+          if (CHECK && last) sPX[t] = qp;
+        }
+        if (fy_on && ly <= RY - 2 - s) {
+          const float qp = py[p] * sE[t] - my[p] * sE[t + RX];
+          qy[p] = qp + wy[p] * (qy[p] - qp);
+          sY[t] = qy[p];
+          if (CHECK && last) sPY[t] = qp;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NSR; ++p) {
+      const int ly = w + NW * p;
+      const int t = ly * RX + lx;
+      if (ly >= s + 1 && ly < RY - 1 - s && c_on && ((dom >> p) & 1u)) {
+        const float cross = ((sX[t + RX] + sX[t + RX - 1]) - (sX[t - RX] + sX[t - RX - 1])) +
+                            ((sY[t + 1] + sY[t - RX + 1]) - (sY[t - 1] + sY[t - RX - 1]));
+        const float D = m0[p] * sX[t] + m1[p] * sX[t - 1] + m2[p] * sY[t] + m3[p] * sY[t - RX] + m4[p] * cross;
+        const float en = e[p] + cc[p] * (rb[p] - D - e[p] * inv_dt);
+        if (CHECK && last) {
+          const float crossp = ((sPX[t + RX] + sPX[t + RX - 1]) - (sPX[t - RX] + sPX[t - RX - 1])) +
+                               ((sPY[t + 1] + sPY[t - RX + 1]) - (sPY[t - 1] + sPY[t - RX - 1]));
+          const float Dp =
+              m0[p] * sPX[t] + m1[p] * sPX[t - 1] + m2[p] * sPY[t] + m3[p] * sPY[t - RX] + m4[p] * crossp;
+          const double res = (double)(rb[p] - en * inv_dt - Dp);
+          sq += res * res;
+          smax = nanmax(smax, fabs(res));
+          if (res != res) smax = res;
+        }
+        e[p] = en;
+        sE[t] = en;
+      }
+    }
+    __syncthreads();
+  }
+
+  const bool cols = lx >= K && lx < K + TX;
+#pragma unroll
+  for (int p = 0; p < NSR; ++p) {
+    const int ly = w + NW * p;
+    const int r = gy + ly;
+    const bool rows = ly >= K && ly < K + TY;
+    const int64_t c = at(g, i, r, 0);
+    if (((dom >> p) & 1u) && rows && cols) {
+      A.e_dst[c] = e[p];
+      A.qx_dst[c] = qx[p];
+      A.qy_dst[c] = qy[p];
+    }
+    if (i == g.lo[0] - 1 && rows && r <= g.hi[1]) A.qx_dst[c] = qx[p];
+    if (r == g.lo[1] - 1 && cols && i <= g.hi[0]) A.qy_dst[c] = qy[p];
+  }
+  if (CHECK) {
+    sq = wave_sum(sq);
+    smax = wave_max(smax);
+    if (lx == 0) {
+      sRed[0][w] = sq;
+      sRed[1][w] = smax;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0, m = 0.0;
+      for (int v = 0; v < NW; ++v) {
+        s += sRed[0][v];
+        m = nanmax(m, sRed[1][v]);
+      }
+      A.part_sum[b] = s;
+      A.part_max[b] = m;
+    }
+  }
+}
+
 template <int K, int TY, int NT>
 static void launch_k(const Lin2DArgs& A, bool check, int nblocks, hipStream_t s) {
   if (check)
@@ -212,14 +385,44 @@ static void launch_k(const Lin2DArgs& A, bool check, int nblocks, hipStream_t s)
     hipLaunchKernelGGL((sweep2d_lin_kernel<K, TY, NT, false>), dim3(nblocks), dim3(NT), 0, s, A);
 }
 
-int lin2d_rows(int kmax) { return kmax >= 4 ? 16 : 8; }
+template <int K, int NW, int NSR>
+static void launch_rows(const Lin2DArgs& A, bool check, int nblocks, hipStream_t s) {
+  if (check)
+    hipLaunchKernelGGL((sweep2d_lin_rows_kernel<K, NW, NSR, true>), dim3(nblocks), dim3(NW * WAVE), 0, s, A);
+  else
+    hipLaunchKernelGGL((sweep2d_lin_rows_kernel<K, NW, NSR, false>), dim3(nblocks), dim3(NW * WAVE), 0, s, A);
+}
 
-void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, bool check, int nblocks, hipStream_t s) {
+// the wave-aligned variant: 8 waves of 4 region rows each (measured at
+// 4096^2: 16 waves of 4 rows, or 8 of 6, run fewer blocks per CU and are
+// 15-35 % slower)
+constexpr int ROWS_NW = 8, ROWS_NSR = 4;
+
+void lin2d_tile(int kmax, int layout, int* cols, int* rows) {
+  if (layout == 0) {
+    *cols = LTX;
+    *rows = kmax >= 4 ? 16 : 8;
+    return;
+  }
+  *cols = WAVE - 2 * kmax;
+  *rows = ROWS_NW * ROWS_NSR - 2 * kmax;
+}
+
+void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, int layout, bool check, int nblocks, hipStream_t s) {
+  if (layout == 0) {
+    switch (kmax) {
+      case 4: launch_k<4, 16, 512>(A, check, nblocks, s); break;
+      case 6: launch_k<6, 16, 1024>(A, check, nblocks, s); break;
+      case 8: launch_k<8, 16, 1024>(A, check, nblocks, s); break;
+      default: launch_k<2, 8, 512>(A, check, nblocks, s); break;
+    }
+    return;
+  }
   switch (kmax) {
-    case 4: launch_k<4, 16, 512>(A, check, nblocks, s); break;
-    case 6: launch_k<6, 16, 1024>(A, check, nblocks, s); break;
-    case 8: launch_k<8, 16, 1024>(A, check, nblocks, s); break;
-    default: launch_k<2, 8, 512>(A, check, nblocks, s); break;
+    case 4: launch_rows<4, ROWS_NW, ROWS_NSR>(A, check, nblocks, s); break;
+    case 6: launch_rows<6, ROWS_NW, ROWS_NSR>(A, check, nblocks, s); break;
+    case 8: launch_rows<8, ROWS_NW, ROWS_NSR>(A, check, nblocks, s); break;
+    default: launch_rows<2, ROWS_NW, ROWS_NSR>(A, check, nblocks, s); break;
   }
 }
 

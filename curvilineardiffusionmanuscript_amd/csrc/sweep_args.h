@@ -107,7 +107,7 @@ void launch_sweep2d(const Sweep2DArgs<T>& A, bool check, int nblocks, hipStream_
 // FP32 correction sweep in coefficient form (sweep2d_lin.hip).
 struct Lin2DArgs {
   Geo g;
-  int ntiles, chunks;  // tiles of 64 x lin2d_rows(kmax) cells; chunks along j
+  int ntiles, chunks;  // tiles of lin2d_tile(kmax, layout) cells; chunks along j
   int nit;             // iterations of this launch, 1..kmax
   int zero_src;        // the correction and its fluxes start from 0 (e_src, q*_src not read)
   float inv_dt;
@@ -127,9 +127,11 @@ struct Lin2DArgs {
   const Ctrl* ctrl;
 };
 
-// kmax: 2, 4, 6 or 8 iterations per launch at most
-int lin2d_rows(int kmax);
-void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, bool check, int nblocks, hipStream_t s);
+// kmax: 2, 4, 6 or 8 iterations per launch at most.  layout 0: tiles of
+// 64 x (8 or 16) cells, region cells spread over the threads; layout 1:
+// region rows aligned to wavefronts, tiles of (64 - 2 kmax) x (32 - 2 kmax)
+void lin2d_tile(int kmax, int layout, int* cols, int* rows);
+void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, int layout, bool check, int nblocks, hipStream_t s);
 
 // FP32 correction sweep in coefficient form, 3D (sweep3d_lin.hip): 64 x 8
 // columns marching over chunks of `ch` planes.

@@ -219,6 +219,37 @@ def test_lin_sweep_depth_does_not_change_result(depth, monkeypatch):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("depth", ["2", "4", "6", "8"])
+@pytest.mark.parametrize("n", [150, 61])
+def test_lin_sweep_row_layout_same_bits(depth, n, monkeypatch):
+    # the wave-aligned sweep (tiles of 64 - 2 depth columns, region rows on
+    # wavefronts) does each face and cell update with the same arithmetic as
+    # the 64-column tiles: same iteration counts and bits as one iteration
+    # per launch on 64-column tiles
+    mesh, T0 = hot_disk_2d(n, seed=5)
+    bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.FixedNegSymmetryBC(), S.NeumannBC()]
+    law = S.PowerLawConductivity(1.0, 2.5, 0.0)
+    out = []
+    monkeypatch.setenv("CDIFF_DC_CHECK", "12")
+    for lay, k, nit in (("1", depth, depth), ("0", "4", "1")):
+        monkeypatch.setenv("CDIFF_LIN2_LAYOUT", lay)
+        monkeypatch.setenv("CDIFF_LIN_K", k)
+        monkeypatch.setenv("CDIFF_LIN_NIT", nit)
+        dev = S.PseudoTransientSolver(mesh, bcs)
+        T = T0.copy(order="F")
+        S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
+        its = []
+        for _ in range(2):
+            st, _ = S.nonlinear_thermal_conduction_step(dev, mesh, T, 1.0, 1.0, law, 1e-5, rtol=0.0, atol=1e-8,
+                                                        max_iter=10**6, precision="fp32_dc",
+                                                        calculate_next_dt=False, show_convergence=False)
+            its.append((st.niter, st.outer, st.abs_err))
+        dev.close()
+        out.append((its, T))
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
 def test_c3_mixed_precision_vs_implicit_direct_512():
     """Mixed precision at 512^2 (config-1 physics: wavy grid A = 0.03, kappa = 1,
     Gaussian T0, zero flux, dt = 1e-4): per-step error against the FP64
