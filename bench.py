@@ -259,7 +259,7 @@ def main():
         scale = float(np.max(np.abs(ref["T_end"])))
         err = float(np.max(np.abs(head["T_end"] - ref["T_end"]))) / scale
         log(f"GPU fp64: {s64 * 1e3:.3f} ms/step, {np.mean(ref['iters']):.1f} its/step; speedup "
-            f"{s64 / s_per_step:.2f}x (without the network {s64 / s_plain:.2f}x); "
+            f"{s64 / s_per_step:.2f}x{f' (without the network {s64 / s_plain:.2f}x)' if lrn is not None else ''}; "
             f"max|T - T_fp64|/max|T| = {err:.2e}")
         extra.update({
             "fp64_s_per_step": round(s64, 6),
