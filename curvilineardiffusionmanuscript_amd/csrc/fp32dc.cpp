@@ -129,18 +129,31 @@ int cdiff_solver::ensure_fp32() {
   // coefficient-form sweep: no periodic faces
   lin = lin_ok();
   if (lin && ndim == 2) {
-    // six iterations per launch on wave-aligned 52 x 20 tiles: the launch
-    // time is mostly its load phase, so deeper launches are cheaper per
-    // iteration (4096^2: 92 us per iteration at depth 4, 78 us at depth 6,
-    // 83 us at depth 8; 100 us at depth 4 on 64 x 16 tiles)
+    // six iterations per launch, marching up strips of 52 columns (4096^2:
+    // 60.5 us per iteration; 80.8 at depth 4, 71.0 at depth 8, whose 183
+    // VGPRs leave two wavefronts per SIMD).  On wave-aligned 52 x 20 tiles
+    // the same launch takes 79 us per iteration (92 at depth 4, 83 at
+    // depth 8; 100 at depth 4 on 64 x 16 tiles): tiles re-read their ring
     lin_k = env_int("CDIFF_LIN_K", 6);
     if (lin_k != 2 && lin_k != 4 && lin_k != 6 && lin_k != 8) lin_k = 6;
     const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
-    lin_layout = env_int("CDIFF_LIN2_LAYOUT", 1) == 0 ? 0 : 1;
+    lin_layout = std::max(0, std::min(2, env_int("CDIFF_LIN2_LAYOUT", 2)));
+    if (lin_layout == 2 && lin_k == 2) lin_layout = 1;  // the marching sweep runs 4, 6 or 8 iterations
     int cols = 64, rows = 16;
-    lin2d_tile(lin_k, lin_layout, &cols, &rows);
+    // (launches of fewer than lin_k iterations use the wave-aligned tiles)
+    lin2d_tile(lin_k, lin_layout == 0 ? 0 : 1, &cols, &rows);
     lin_chunks = (nj + rows - 1) / rows;
     lin_tiles = ((ni + cols - 1) / cols) * lin_chunks;
+    if (lin_layout == 2) {
+      // marching sweep: strips of 64 - 2 lin_k columns; chunks of rows sized
+      // for about 12 wavefronts per CU, at least 2 lin_k rows each
+      const int strips = (ni + 63 - 2 * lin_k) / (64 - 2 * lin_k);
+      const int want = std::max(1, 12 * ncu / strips);
+      march_rows = std::max(2 * lin_k, (nj + want - 1) / want);
+      march_rows = std::max(2 * lin_k, env_int("CDIFF_MARCH_ROWS", march_rows));
+      march_chunks = (nj + march_rows - 1) / march_rows;
+      march_items = strips * march_chunks;
+    }
   } else if (lin) {
     // 3D: one or two iterations per launch; 64 x 8 columns over chunks of planes
     lin_k = std::max(1, std::min(2, env_int("CDIFF_LIN3_K", 2)));
@@ -161,7 +174,8 @@ int cdiff_solver::ensure_fp32() {
   }
   if (lin) {
     lin_nblocks = (lin_tiles + 7) / 8 * 8;
-    const int nparts_need = std::max(lin_nblocks, lin3b_nblocks);
+    const int march_parts = (ndim == 2 && lin_layout == 2) ? lin2d_march_parts(march_items) : 0;
+    const int nparts_need = std::max(std::max(lin_nblocks, lin3b_nblocks), march_parts);
     // one block each: the folded metrics m[c] at m[0] + c * total; the face
     // coefficients [axis][P, M, W] at fx[0] + (3 axis + c) * total
     auto alloc_block = [&](float** p, int n) -> int {
@@ -388,7 +402,16 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit, bool zero_
   A.part_sum = lin_psum;
   A.part_max = lin_pmax;
   A.ctrl = ctrl_in;
-  launch_sweep2d_lin(A, lin_k, lin_layout, check, lin_nblocks, stream);
+  if (lin_layout == 2 && nit == lin_k) {
+    A.ntiles = march_items;
+    A.chunks = march_chunks;
+    A.rows_per_chunk = march_rows;
+    launch_sweep2d_march(A, lin_k, check, stream);
+    lin_last_parts = lin2d_march_parts(march_items);
+    LAUNCH_CHECK("sweep2d_march");
+    return CDIFF_OK;
+  }
+  launch_sweep2d_lin(A, lin_k, lin_layout == 0 ? 0 : 1, check, lin_nblocks, stream);
   lin_last_parts = lin_nblocks;
   LAUNCH_CHECK("sweep2d_lin");
   return CDIFF_OK;

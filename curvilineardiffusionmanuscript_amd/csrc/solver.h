@@ -65,7 +65,8 @@ struct cdiff_solver {
   // FP32 correction sweep in coefficient form (2D, no periodic faces)
   bool lin = false;
   int lin_k = 4;  // iterations per launch at most
-  int lin_layout = 1;  // 2D sweep tile layout (lin2d_tile)
+  int lin_layout = 2;  // 2D sweep: 0, 1 tile layouts (lin2d_tile), 2 marching
+  int march_rows = 0, march_chunks = 0, march_items = 0;  // marching sweep work split
   int lin_tiles = 0, lin_chunks = 0, lin_nblocks = 0;
   float* lin_m[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   float* lin_fx[3] = {nullptr, nullptr, nullptr};

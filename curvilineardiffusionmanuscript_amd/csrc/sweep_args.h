@@ -109,6 +109,7 @@ struct Lin2DArgs {
   Geo g;
   int ntiles, chunks;  // tiles of lin2d_tile(kmax, layout) cells; chunks along j
   int nit;             // iterations of this launch, 1..kmax
+  int rows_per_chunk;  // marching sweep: rows of a chunk (ntiles = strips x chunks)
   int zero_src;        // the correction and its fluxes start from 0 (e_src, q*_src not read)
   float inv_dt;
   const float* __restrict__ e_src;
@@ -132,6 +133,11 @@ struct Lin2DArgs {
 // region rows aligned to wavefronts, tiles of (64 - 2 kmax) x (32 - 2 kmax)
 void lin2d_tile(int kmax, int layout, int* cols, int* rows);
 void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, int layout, bool check, int nblocks, hipStream_t s);
+// the marching sweep (sweep2d_march.hip): strips of 64 - 2 kmax columns, one
+// wavefront per strip and chunk of rows, exactly kmax iterations per launch;
+// partial sums: lin2d_march_parts(ntiles) entries
+int lin2d_march_parts(int ntiles);
+void launch_sweep2d_march(const Lin2DArgs& A, int kmax, bool check, hipStream_t s);
 
 // FP32 correction sweep in coefficient form, 3D (sweep3d_lin.hip): 64 x 8
 // columns marching over chunks of `ch` planes.

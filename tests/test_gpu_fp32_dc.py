@@ -219,19 +219,23 @@ def test_lin_sweep_depth_does_not_change_result(depth, monkeypatch):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("depth", ["2", "4", "6", "8"])
+@pytest.mark.parametrize("layout,depth,rows", [("1", "2", ""), ("1", "4", ""), ("1", "6", ""), ("1", "8", ""),
+                                               ("2", "4", ""), ("2", "6", ""), ("2", "8", ""), ("2", "6", "17")])
 @pytest.mark.parametrize("n", [150, 61])
-def test_lin_sweep_row_layout_same_bits(depth, n, monkeypatch):
-    # the wave-aligned sweep (tiles of 64 - 2 depth columns, region rows on
-    # wavefronts) does each face and cell update with the same arithmetic as
-    # the 64-column tiles: same iteration counts and bits as one iteration
-    # per launch on 64-column tiles
+def test_lin_sweep_row_layout_same_bits(layout, depth, rows, n, monkeypatch):
+    # the wave-aligned sweep (layout 1: tiles of 64 - 2 depth columns, region
+    # rows on wavefronts) and the marching sweep (layout 2: one wavefront per
+    # strip of 64 - 2 depth columns and chunk of rows, the iterations
+    # pipelined up the rows) do each face and cell update with the same
+    # arithmetic as the 64-column tiles: same iteration counts and bits as
+    # one iteration per launch on 64-column tiles
     mesh, T0 = hot_disk_2d(n, seed=5)
     bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.FixedNegSymmetryBC(), S.NeumannBC()]
     law = S.PowerLawConductivity(1.0, 2.5, 0.0)
     out = []
     monkeypatch.setenv("CDIFF_DC_CHECK", "12")
-    for lay, k, nit in (("1", depth, depth), ("0", "4", "1")):
+    monkeypatch.setenv("CDIFF_MARCH_ROWS", rows)
+    for lay, k, nit in ((layout, depth, depth), ("0", "4", "1")):
         monkeypatch.setenv("CDIFF_LIN2_LAYOUT", lay)
         monkeypatch.setenv("CDIFF_LIN_K", k)
         monkeypatch.setenv("CDIFF_LIN_NIT", nit)
