@@ -69,6 +69,15 @@ __device__ __forceinline__ void load_row(const Lin2DArgs& A, int i, int r, bool 
   }
 }
 
+// east / west neighbour by a DPP wavefront shift by one lane (the edge lane
+// reads 0; edge lanes lie outside every written cell's dependency cone)
+__device__ __forceinline__ float dpp_from_left(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));  // wave_shr:1
+}
+__device__ __forceinline__ float dpp_from_right(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));  // wave_shl:1
+}
+
 }  // namespace
 
 template <int K, bool CHECK>
@@ -168,7 +177,7 @@ __global__ __launch_bounds__(MWAVES * WAVE) void sweep2d_lin_march_kernel(Lin2DA
       // x faces of row j + 1
       {
         const float e0 = E[n - 1][0];
-        const float e1 = from_right(e0);
+        const float e1 = dpp_from_right(e0);
         const float qold = n == 1 ? QX[0][0] : QX[n - 1][1];
         const float qp = FX[n - 1][0] * e0 - FX[n - 1][1] * e1;
         QX[n][0] = qp + FX[n - 1][2] * (qold - qp);
@@ -186,8 +195,8 @@ __global__ __launch_bounds__(MWAVES * WAVE) void sweep2d_lin_march_kernel(Lin2DA
       {
         const float xc = QX[n][1], xn = QX[n][0], xs = QX[n][2];
         const float yc = QY[n][0], ys = QY[n][1];
-        const float xw = from_left(xc), xnw = from_left(xn), xsw = from_left(xs);
-        const float ye = from_right(yc), yes = from_right(ys), yw = from_left(yc), yws = from_left(ys);
+        const float xw = dpp_from_left(xc), xnw = dpp_from_left(xn), xsw = dpp_from_left(xs);
+        const float ye = dpp_from_right(yc), yes = dpp_from_right(ys), yw = dpp_from_left(yc), yws = dpp_from_left(ys);
         const float cross = ((xn + xnw) - (xs + xsw)) + ((ye + yes) - (yw + yws));
         const float D = CY[n][5] * xc + CY[n][6] * xw + CY[n][7] * yc + CY[n][8] * ys + CY[n][9] * cross;
         const float eo = E[n - 1][1];
@@ -199,8 +208,8 @@ __global__ __launch_bounds__(MWAVES * WAVE) void sweep2d_lin_march_kernel(Lin2DA
           // the strip's domain cells of the chunk's rows are summed)
           const float pxc = QPX[1], pxn = QPX[0], pxs = QPX[2];
           const float pyc = QPY[0], pys = QPY[1];
-          const float pxw = from_left(pxc), pxnw = from_left(pxn), pxsw = from_left(pxs);
-          const float pye = from_right(pyc), pyes = from_right(pys), pyw = from_left(pyc), pyws = from_left(pys);
+          const float pxw = dpp_from_left(pxc), pxnw = dpp_from_left(pxn), pxsw = dpp_from_left(pxs);
+          const float pye = dpp_from_right(pyc), pyes = dpp_from_right(pys), pyw = dpp_from_left(pyc), pyws = dpp_from_left(pys);
           const float crossp = ((pxn + pxnw) - (pxs + pxsw)) + ((pye + pyes) - (pyw + pyws));
           const float Dp = CY[n][5] * pxc + CY[n][6] * pxw + CY[n][7] * pyc + CY[n][8] * pys + CY[n][9] * crossp;
           const double res = (dom && cols && j >= ya && j < yb) ? (double)(CY[n][3] - en * inv_dt - Dp) : 0.0;
