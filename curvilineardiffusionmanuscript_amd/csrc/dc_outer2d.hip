@@ -23,7 +23,7 @@ namespace cdiff {
 
 namespace {
 
-constexpr int OTX = 64, OTY = 8;
+constexpr int OTX = 64, OTY = DC_OUTER2D_TY;
 constexpr int OCX = OTX + 2, OCY = OTY + 2, ONC = OCX * OCY;
 constexpr int OFXN = OTX + 1, OFYN = OTX + 2;
 constexpr int ONFX = OFXN * OCY, ONFY = OFYN * (OTY + 1);

@@ -488,7 +488,7 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
   A.theta_scale = dc_theta_scale;
   A.harmonic = mean == CDIFF_MEAN_HARMONIC;
   A.newton = dc_newton;
-  A.chunks = (nj + 7) / 8;
+  A.chunks = (nj + DC_OUTER2D_TY - 1) / DC_OUTER2D_TY;
   A.ntiles = ((ni + 63) / 64) * A.chunks;
   A.t_exp = kap.t_exp;
   A.rho = rho;
@@ -511,7 +511,7 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
   A.pmax = dc_pmax;
   const int nblk = (A.ntiles + 7) / 8 * 8;
   if (nblk > dc_nparts) return fail(CDIFF_HIP_ERROR, "partial-sum buffer too small");
-  hipLaunchKernelGGL(k_dc_outer2d, dim3(nblk), dim3(512), 0, stream, A);
+  hipLaunchKernelGGL(k_dc_outer2d, dim3(nblk), dim3(64 * DC_OUTER2D_TY), 0, stream, A);
   LAUNCH_CHECK("k_dc_outer2d");
   if (finalize) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl, (const double*)dc_psum,
@@ -526,7 +526,7 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
 int cdiff_solver::dc_finalize_outer(int iter, const cdiff_opts& o) {
   if (ndim != 2 || !lin) return fail(CDIFF_BAD_ARG, "no 2D outer pass to finalize");
   const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
-  const int ntiles = ((ni + 63) / 64) * ((nj + 7) / 8);
+  const int ntiles = ((ni + 63) / 64) * ((nj + DC_OUTER2D_TY - 1) / DC_OUTER2D_TY);
   const int nblk = (ntiles + 7) / 8 * 8;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl, (const double*)dc_psum, (const double*)dc_pmax,
                      nblk, iter, INT_MAX, o.rtol, o.atol, N_full, o.norm_kind, 1);
