@@ -374,7 +374,7 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   const bool warm_here = net && (o.precision == CDIFF_PREC_FP32_DC || o.precision == CDIFF_PREC_FP64) && ndim == 2 &&
                          lin_ok();
   // after predictions that lost to T^n or won by less than half, the network
-  // sits out 1, 3, 7 (at most 8) steps before it is consulted again
+  // sits out 3, 7, 15 (at most 16) steps before it is consulted again
   const bool warm_skipped = warm_here && warm_skip > 0;
   if (warm_skipped) --warm_skip;
   if (warm_here && !warm_skipped) {
