@@ -147,12 +147,13 @@ int cdiff_solver::ensure_fp32() {
     if (lin_layout == 2) {
       // marching sweep: strips of 64 - 2 lin_k columns; chunks of rows sized
       // for about 12 wavefronts per CU, at least 2 lin_k rows each
-      const int strips = (ni + 63 - 2 * lin_k) / (64 - 2 * lin_k);
-      const int want = std::max(1, 12 * ncu / strips);
+      // (a multiple of 8 chunks when there are that many: one set per XCD)
+      march_strips = (ni + 63 - 2 * lin_k) / (64 - 2 * lin_k);
+      int want = std::max(1, 12 * ncu / march_strips);
+      if (want >= 8) want -= want % 8;
       march_rows = std::max(2 * lin_k, (nj + want - 1) / want);
       march_rows = std::max(2 * lin_k, env_int("CDIFF_MARCH_ROWS", march_rows));
       march_chunks = (nj + march_rows - 1) / march_rows;
-      march_items = strips * march_chunks;
     }
   } else if (lin) {
     // 3D: one or two iterations per launch; 64 x 8 columns over chunks of planes
@@ -174,7 +175,7 @@ int cdiff_solver::ensure_fp32() {
   }
   if (lin) {
     lin_nblocks = (lin_tiles + 7) / 8 * 8;
-    const int march_parts = (ndim == 2 && lin_layout == 2) ? lin2d_march_parts(march_items) : 0;
+    const int march_parts = (ndim == 2 && lin_layout == 2) ? lin2d_march_parts(march_strips, march_chunks) : 0;
     const int nparts_need = std::max(std::max(lin_nblocks, lin3b_nblocks), march_parts);
     // one block each: the folded metrics m[c] at m[0] + c * total; the face
     // coefficients [axis][P, M, W] at fx[0] + (3 axis + c) * total
@@ -403,11 +404,11 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit, bool zero_
   A.part_max = lin_pmax;
   A.ctrl = ctrl_in;
   if (lin_layout == 2 && nit == lin_k) {
-    A.ntiles = march_items;
+    A.strips = march_strips;
     A.chunks = march_chunks;
     A.rows_per_chunk = march_rows;
     launch_sweep2d_march(A, lin_k, check, stream);
-    lin_last_parts = lin2d_march_parts(march_items);
+    lin_last_parts = lin2d_march_parts(march_strips, march_chunks);
     LAUNCH_CHECK("sweep2d_march");
     return CDIFF_OK;
   }

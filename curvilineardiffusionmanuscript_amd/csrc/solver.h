@@ -66,7 +66,7 @@ struct cdiff_solver {
   bool lin = false;
   int lin_k = 4;  // iterations per launch at most
   int lin_layout = 2;  // 2D sweep: 0, 1 tile layouts (lin2d_tile), 2 marching
-  int march_rows = 0, march_chunks = 0, march_items = 0;  // marching sweep work split
+  int march_rows = 0, march_chunks = 0, march_strips = 0;  // marching sweep work split
   int lin_tiles = 0, lin_chunks = 0, lin_nblocks = 0;
   float* lin_m[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   float* lin_fx[3] = {nullptr, nullptr, nullptr};

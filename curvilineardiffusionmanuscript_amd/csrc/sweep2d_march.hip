@@ -86,15 +86,23 @@ __global__ __launch_bounds__(MWAVES * WAVE) void sweep2d_lin_march_kernel(Lin2DA
   if (A.ctrl->done) return;
   const Geo& g = A.g;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * MWAVES + (threadIdx.x >> 6));
-  if (item >= A.ntiles) {
+  // XCD-aware split: block b runs on XCD b mod 8; the blocks of one XCD take
+  // the chunks c = XCD, XCD + 8, ... and, per chunk, groups of MWAVES
+  // neighbouring strips in order, so strips that share the cache lines at
+  // their edges share that XCD's L2 (they march up the same rows together)
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int b = blockIdx.x, k = b >> 3;
+  const int groups = (A.strips + MWAVES - 1) / MWAVES;
+  const int chunk = (b & 7) + 8 * (k / groups);
+  const int strip = (k % groups) * MWAVES + w;
+  const int part = b * MWAVES + w;
+  if (chunk >= A.chunks || strip >= A.strips) {
     if (CHECK && lane == 0) {
-      A.part_sum[item] = 0.0;
-      A.part_max[item] = 0.0;
+      A.part_sum[part] = 0.0;
+      A.part_max[part] = 0.0;
     }
     return;
   }
-  const int strip = item / A.chunks, chunk = item - strip * A.chunks;
   const int x0 = g.lo[0] + strip * TX;
   const int i = x0 - K + lane;
   const int ya = g.lo[1] + chunk * A.rows_per_chunk;
@@ -234,16 +242,20 @@ __global__ __launch_bounds__(MWAVES * WAVE) void sweep2d_lin_march_kernel(Lin2DA
     sq = wave_sum(sq);
     smax = wave_max(smax);
     if (lane == 0) {
-      A.part_sum[item] = sq;
-      A.part_max[item] = smax;
+      A.part_sum[part] = sq;
+      A.part_max[part] = smax;
     }
   }
 }
 
-int lin2d_march_parts(int ntiles) { return (ntiles + MWAVES - 1) / MWAVES * MWAVES; }
+int lin2d_march_blocks(int strips, int chunks) {
+  return 8 * ((strips + MWAVES - 1) / MWAVES) * ((chunks + 7) / 8);
+}
+
+int lin2d_march_parts(int strips, int chunks) { return MWAVES * lin2d_march_blocks(strips, chunks); }
 
 void launch_sweep2d_march(const Lin2DArgs& A, int kmax, bool check, hipStream_t s) {
-  const int nb = (A.ntiles + MWAVES - 1) / MWAVES;
+  const int nb = lin2d_march_blocks(A.strips, A.chunks);
   switch (kmax) {
     case 4:
       if (check)

@@ -109,7 +109,8 @@ struct Lin2DArgs {
   Geo g;
   int ntiles, chunks;  // tiles of lin2d_tile(kmax, layout) cells; chunks along j
   int nit;             // iterations of this launch, 1..kmax
-  int rows_per_chunk;  // marching sweep: rows of a chunk (ntiles = strips x chunks)
+  int rows_per_chunk;  // marching sweep: rows of a chunk
+  int strips;          // marching sweep: strips of 64 - 2 kmax columns
   int zero_src;        // the correction and its fluxes start from 0 (e_src, q*_src not read)
   float inv_dt;
   const float* __restrict__ e_src;
@@ -135,8 +136,9 @@ void lin2d_tile(int kmax, int layout, int* cols, int* rows);
 void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, int layout, bool check, int nblocks, hipStream_t s);
 // the marching sweep (sweep2d_march.hip): strips of 64 - 2 kmax columns, one
 // wavefront per strip and chunk of rows, exactly kmax iterations per launch;
-// partial sums: lin2d_march_parts(ntiles) entries
-int lin2d_march_parts(int ntiles);
+// partial sums: lin2d_march_parts(strips, chunks) entries
+int lin2d_march_blocks(int strips, int chunks);
+int lin2d_march_parts(int strips, int chunks);
 void launch_sweep2d_march(const Lin2DArgs& A, int kmax, bool check, hipStream_t s);
 
 // FP32 correction sweep in coefficient form, 3D (sweep3d_lin.hip): 64 x 8
