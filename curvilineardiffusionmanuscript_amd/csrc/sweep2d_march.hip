@@ -10,11 +10,13 @@
 //   y faces of row t - n       q'_y = P e - M e_north, q_y <- q' + W (q_y - q')
 //   cells   of row t - n       e <- e + C (r - D - e/dt)
 // from the iteration n - 1 values of the rows around it (kept in registers,
-// east and west neighbours by wave shuffles).  Stage n needs iteration n - 1
+// east and west neighbours by DPP wavefront shifts).  Stage n needs iteration n - 1
 // of row t - n + 1, which stage n - 1 produced in the same step, so the
 // stages run in order.  Each row's coefficients stay in registers for the K
 // steps that use them.  Against the tiled sweep, the ring is re-read only
-// along x (64 / (64 - 2K)) and 2K rows per chunk, and there are no barriers.
+// along x (64 / (64 - 2K)) and 2K rows per chunk, and there are no barriers;
+// the neighbouring strips of a chunk run on one XCD and share the cache lines
+// at their edges in its L2.
 //
 // Values outside the dependency cone of the strip's cells (the ring's edge
 // lanes and the first rows of a chunk) are computed from incomplete
