@@ -130,7 +130,7 @@ int cdiff_solver::ensure_fp32() {
   lin = lin_ok();
   if (lin && ndim == 2) {
     // six iterations per launch, marching up strips of 52 columns (4096^2:
-    // 60.5 us per iteration; 80.8 at depth 4, 71.0 at depth 8, whose 183
+    // 57-58 us per iteration; 80.8 at depth 4, 71.0 at depth 8, whose 183
     // VGPRs leave two wavefronts per SIMD).  On wave-aligned 52 x 20 tiles
     // the same launch takes 79 us per iteration (92 at depth 4, 83 at
     // depth 8; 100 at depth 4 on 64 x 16 tiles): tiles re-read their ring
