@@ -46,10 +46,10 @@ struct Ctrl {
   int status;      // cdiff_status of the loop
   int final_iter;  // iteration at which it ended
   int nchecks;
-  double rel, abs_, init;
+  double rel, abs_, init;  // (inner correction loop: init = the smallest residual of its checks so far)
   double last_l2;
   int valid_flags;  // step-start validity (bit 0: alpha domain, bits 1..: halo sides; bit 16+: source)
-  int pad;
+  int grow;         // inner correction loop: consecutive checks whose residual grew
 };
 
 // Julia `x * isfinite(x)` (Bool strong zero).

@@ -681,6 +681,9 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
       HIP_TRY(hipEventRecord(ev[seg & 1], stream));
       if (pending) {
         HIP_TRY(hipEventSynchronize(ev[(seg - 1) & 1]));
+        if (dc_debug)
+          std::fprintf(stderr, "[dc]   inner check %d: residual %.3e\n", ctrl_h[(seg - 1) & 1].nchecks,
+                       ctrl_h[(seg - 1) & 1].last_l2);
         if (ctrl_h[(seg - 1) & 1].done) break;
       }
       if (it >= cap) break;

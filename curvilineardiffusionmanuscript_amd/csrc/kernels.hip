@@ -197,6 +197,7 @@ __global__ void k_ctrl_reset(Ctrl* ctrl, double rtol, double atol) {
   ctrl->init = INFINITY;
   ctrl->last_l2 = 0.0;
   ctrl->valid_flags = 0;
+  ctrl->grow = 0;
 }
 
 __global__ void k_ctrl_validated(Ctrl* ctrl) {

@@ -308,3 +308,30 @@ def test_unstable_pseudo_time_step_falls_back(monkeypatch):
         res[scale] = T
         dev.close()
     assert np.max(np.abs(res["3.0"] - res["1.0"])) <= 2 * 1e-5 * np.sqrt(T0.size) * 1e-8 * 4
+
+
+def test_slowly_diverging_correction_solve_is_dropped_early():
+    """A correction solve whose residual first falls and then grows slowly
+    (1.1x per 12 sweeps under the enlarged pseudo-time step) is abandoned
+    after a few checks of growth, not after the 100x bound: on the 4096^2
+    blob problem (kappa = T^1.77, held-out seed 0 of the warm-start
+    evaluation) the fourth step took 1620 inner sweeps before, 336 now; every
+    step still ends at the FP64 tolerance."""
+    from curvilineardiffusionmanuscript_amd import learned as LW
+
+    n = 4096
+    rng = np.random.default_rng(10**6)
+    mesh = LW.crop_mesh(n, n, (0.0, 0.0))
+    T0 = LW.blob_temperature(mesh, rng, int(rng.integers(1, 17)))
+    law = S.PowerLawConductivity(1.0, float(rng.uniform(1.0, 3.0)), 0.0)
+    sol = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4)
+    S.update_conductivity(sol, mesh, T0, 1.0, 1.0, law)
+    sol.set_temperature(T0)
+    dt, its = 1e-6, []
+    for _ in range(4):
+        st, dt = sol.step_resident(dt, None, 1.0, law, rtol=0.0, atol=1e-8, max_iter=10**7, precision="fp32_dc")
+        its.append(st.niter)
+        assert st.abs_err <= 1e-8 and not st.fp64_fallback
+    sol.close()
+    print("inner sweeps per step", its)
+    assert max(its) <= 600
