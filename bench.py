@@ -22,12 +22,21 @@ method the metric names (FP32 + learned); in 3D, where the warm start does
 not apply, of the mixed-precision step.
 
 Prints one JSON line: s/time-step (lower is better), with
-  roofline      the dominant kernel's algorithmic bytes per launch (cells x
-                pseudo-iterations x bytes per cell-iteration) / its mean
-                launch duration (HIP events on the solver's stream), against
-                8 TB/s; `traffic` is the measured HBM bytes per launch
-  cpu_baseline  the oracle (CPU restatement of the reference APT, OpenMP)
-                on a bounded sample of the same workload on this host
+  roofline      the dominant kernel's compulsory HBM bytes per launch (every
+                array the launch needs read once and every array it updates
+                written once: cells x bytes per cell per launch, whatever the
+                number of pseudo-iterations the launch runs) / its mean launch
+                duration (HIP events on the solver's stream), against 8 TB/s;
+                `t_eff` is the same launch under the APT effective-throughput
+                convention (A_eff = (2 D_u + D_k) bytes per cell and
+                pseudo-iteration, charged once per iteration), `traffic` the
+                HBM bytes per launch rocprofv3's counters measured (source and
+                date in `traffic_source`)
+  cpu_baseline  the oracle (CPU restatement of the reference APT, OpenMP on
+                OMP_NUM_THREADS threads) on the first steps of the same
+                workload, with the GPU's times for the same steps beside it;
+                in 2D also the error of the mixed-precision step against the
+                FP64 implicit-direct oracle on the 512^2 Gaussian subsample
 """
 from __future__ import annotations
 
@@ -45,7 +54,23 @@ for p in (ROOT, os.path.join(ROOT, "tests")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+# A_eff of the APT effective-throughput convention, bytes per cell and
+# pseudo-iteration: (2 D_u + D_k) x bytes per value
 BYTES_PER_CELL = {("fp64", 2): 96, ("fp32_dc", 2): 48, ("fp64", 3): 144, ("fp32_dc", 3): 72}
+# compulsory bytes per cell and launch of the dominant kernels: what one launch
+# must read (coefficients, sources, iterates) and write (iterates) once, for all
+# the iterations it runs.
+#   FP64 2D, two iterations: u, q_xi, q_eta read and written (48), u_prev and
+#     g11, g22, c12, alpha_c, beta_c read (48)
+#   FP32 2D correction sweep, six iterations: e, q_xi, q_eta read and written
+#     (24), five folded metric terms, P, M, W of both face sets, C and the
+#     right-hand side read (52)
+#   FP64 3D, one iteration: u and three fluxes read and written (64), u_prev and
+#     nine coefficients read (80)
+#   FP32 3D correction sweep, two iterations: e and three fluxes read and
+#     written (32), nine folded metric terms, P, M, W of three face sets, C and
+#     the right-hand side read (80)
+COMPULSORY_BYTES_PER_CELL = {("fp64", 2): 96, ("fp32_dc", 2): 76, ("fp64", 3): 144, ("fp32_dc", 3): 112}
 
 
 # workloads: mesh and T0 builder, boundary conditions (device, oracle), kappa
@@ -101,7 +126,7 @@ def run_gpu(args, mesh, T0, method, net=None):
         st, dt = sol.step_resident(dt, None, 1.0, law, **kw)
     reset()
     dt = args.dt0
-    iters, checks, steps, outer, warm = [], 0, [], [], 0
+    iters, checks, steps, outer, warm, step_times = [], 0, [], [], 0, []
     t0 = time.perf_counter()  # step_resident returns after the step's work has finished
     for k in range(args.steps):
         st, dt_next = sol.step_resident(dt, None, 1.0, law, **kw)
@@ -110,6 +135,7 @@ def run_gpu(args, mesh, T0, method, net=None):
         checks += st.nchecks
         warm += int(st.warm_start)
         steps.append((dt, st.niter, st.abs_err, st.time))
+        step_times.append(st.time)
         dt = dt_next
     elapsed = time.perf_counter() - t0
     for k, (d, it, err, tm) in enumerate(steps[: min(4, len(steps))]):
@@ -120,7 +146,7 @@ def run_gpu(args, mesh, T0, method, net=None):
         mean_ms = sol_bench(sol, steps[-1][0], method, args.sweep_samples)
     sol.close()
     return dict(elapsed=elapsed, iters=iters, outer=outer, checks=checks, T_end=T_end, sweep_ms=mean_ms,
-                abs_err=[s[2] for s in steps], warm_used=warm)
+                abs_err=[s[2] for s in steps], warm_used=warm, step_times=step_times)
 
 
 # dominant kernel of each method and dimension, with the pseudo-iterations
@@ -148,11 +174,13 @@ def sol_bench(sol, dt, method, n):
 
 
 def run_cpu(args, mesh, T0, budget_s):
-    """The oracle on the first steps of the same workload, up to ~budget_s."""
+    """The oracle on the first steps of the same workload, up to ~budget_s.
+    Threads: OMP_NUM_THREADS (the host's CPU share per GPU where it is set),
+    else every core os.cpu_count() reports."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import apt_oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     O.set_num_threads(threads)
     w = WORKLOADS[args.workload]
     s = O.OracleSolver(mesh, w["bcs"], kappa=w["kappa"])
@@ -173,12 +201,53 @@ def run_cpu(args, mesh, T0, budget_s):
 
 def load_traffic(kernel, n, ndim):
     """Measured HBM bytes per launch of `kernel` at n^ndim (rocprofv3 counters,
-    calibrated against a copy kernel), if profiles/traffic.json has them."""
+    calibrated against a copy kernel) and where they come from, if
+    profiles/traffic.json has them."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get(f"{kernel} @ {'x'.join([str(n)] * ndim)}")
-    return None
+            d = json.load(f)
+        e = d.get(f"{kernel} @ {'x'.join([str(n)] * ndim)}")
+        if e:
+            src = d.get("_source", {})
+            return e, f"{src.get('file', 'profiles/traffic.json')}, measured {e.get('measured', src.get('measured', '?'))}"
+    return None, None
+
+
+def implicit_oracle_check(n=512, dt=1e-4, steps=2):
+    """Error of the mixed-precision step against the FP64 implicit-direct
+    oracle (oracle/implicit_oracle.py: the reference's conservative scheme,
+    SciPy direct solve) on the Gaussian subsample: wavy grid A = 0.03, kappa = 1,
+    zero-flux faces, dt = 1e-4.  The two are different consistent
+    discretisations, so the figure is their difference over the central half of
+    the domain relative to the step's increment there (O(h^2))."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import implicit_oracle as I
+    from cases import gaussian_2d
+
+    from curvilineardiffusionmanuscript_amd import solver as S
+
+    mesh, T0 = gaussian_2d(n)
+    law = S.PowerLawConductivity(1.0)
+    dev = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4)
+    imp = I.ImplicitOracle2D(mesh, [("neumann", 0.0)] * 4)
+    Td, Ti = T0.copy(order="F"), T0.copy(order="F")
+    S.update_conductivity(dev, mesh, Td, 1.0, 1.0, law)
+    q = slice(1 + n // 4, 1 + 3 * n // 4)
+    errs, rels = [], []
+    for _ in range(steps):
+        Tprev = Td.copy()
+        S.nonlinear_thermal_conduction_step(dev, mesh, Td, 1.0, 1.0, law, dt, rtol=0.0, atol=1e-8, max_iter=10**6,
+                                            precision="fp32_dc", calculate_next_dt=False, show_convergence=False)
+        imp.step(Ti, dt, cutoff=False)
+        err = float(np.max(np.abs(Td - Ti)[q, q]))
+        errs.append(err)
+        rels.append(err / float(np.max(np.abs(Td - Tprev)[q, q])))
+        Ti[...] = Td
+    dev.close()
+    return {"workload": f"2D {n}x{n} Gaussian, wavy grid A=0.03, kappa=1, zero flux, dt={dt:g}, {steps} steps",
+            "max_abs_err_per_step": errs, "max_err_over_step_increment": rels,
+            "region": "central half of the domain (the two schemes differ in the boundary band)"}
 
 
 def main():
@@ -198,6 +267,8 @@ def main():
                     help="skip the run with the learned warm start (2D mixed precision only)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-implicit", action="store_true",
+                    help="skip the 512^2 comparison with the FP64 implicit-direct oracle")
     ap.add_argument("--sweep-samples", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -275,9 +346,11 @@ def main():
     kname, per_launch, _ = KERNELS[(args.method, ndim)]
     per_it_ms = g["sweep_ms"]
     launch_ms = per_it_ms * per_launch if per_it_ms == per_it_ms else None
-    alg_bytes = bpc * n_cells * per_launch
-    achieved = alg_bytes / (launch_ms / 1e3) / 1e9 if launch_ms else None
-    traffic = load_traffic(kname, args.n, ndim)
+    comp_bytes = COMPULSORY_BYTES_PER_CELL[(args.method, ndim)] * n_cells
+    aeff_bytes = bpc * n_cells * per_launch
+    achieved = comp_bytes / (launch_ms / 1e3) / 1e9 if launch_ms else None
+    t_eff = aeff_bytes / (launch_ms / 1e3) / 1e9 if launch_ms else None
+    traffic, traffic_src = load_traffic(kname, args.n, ndim)
     roofline = {
         "kernel": kname,
         "bound": "hbm",
@@ -286,24 +359,42 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
         "traffic": traffic.get("bytes_per_launch") if traffic else None,
-        "algorithmic_bytes_per_launch": alg_bytes,
+        "traffic_source": traffic_src,
+        "compulsory_bytes_per_launch": comp_bytes,
+        "compulsory_bytes_per_cell": COMPULSORY_BYTES_PER_CELL[(args.method, ndim)],
         "iterations_per_launch": per_launch,
         "mean_launch_us": round(launch_ms * 1e3, 2) if launch_ms else None,
+        "t_eff": {"convention": f"A_eff = {bpc} B per cell and pseudo-iteration x {per_launch} iterations per launch",
+                  "GBs": round(t_eff, 1) if t_eff else None,
+                  "frac": round(t_eff / HBM_PEAK_GBS, 4) if t_eff else None},
     }
     if traffic and launch_ms:
-        roofline["hbm_frac_measured"] = round(traffic["bytes_per_launch"] / (launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+        roofline["traffic_GBs"] = round(traffic["bytes_per_launch"] / (launch_ms / 1e3) / 1e9, 1)
     cpu = None
     if not args.no_cpu:
         c = run_cpu(args, mesh, T0, args.cpu_seconds)
-        log(f"CPU oracle: {c['s_per_step']:.3f} s/step over {c['steps']} steps, {c['threads']} threads")
+        k = c["steps"]
+        same = {"gpu_" + name + "_s_per_step": round(float(np.mean(r["step_times"][:k])), 6)
+                for name, r in (("head", head), ("fp64", ref)) if r is not None}
+        gpu_it = (float(np.sum(ref["step_times"][:k])) / float(np.sum(ref["iters"][:k])) * 1e3) if ref else None
+        log(f"CPU oracle: {c['s_per_step']:.3f} s/step over {k} steps ({c['iters']} iterations), {c['threads']} "
+            f"threads; the same steps on the GPU: {same}")
         cpu = {
             "value": round(c["s_per_step"], 5),
             "unit": "s/time-step",
             "cores": c["threads"],
             "kind": "port",
-            "sample": f"first {c['steps']} of the {args.steps} steps ({c['iters']} pseudo-iterations), "
-            f"same inputs and tolerance, FP64, oracle/apt_oracle.c with OpenMP",
+            "sample": f"first {k} of the {args.steps} steps ({c['iters']} pseudo-iterations), same inputs and "
+            f"tolerance, FP64, oracle/apt_oracle.c with OpenMP on OMP_NUM_THREADS threads",
+            "same_steps": same,
+            "ms_per_iteration": round(c["elapsed"] / c["iters"] * 1e3, 4),
+            "gpu_fp64_ms_per_iteration": round(gpu_it, 4) if gpu_it else None,
         }
+        if ndim == 2 and not args.no_implicit:
+            t_i = time.perf_counter()
+            extra["error_vs_implicit_oracle"] = implicit_oracle_check()
+            log(f"implicit-direct oracle check: {extra['error_vs_implicit_oracle']['max_err_over_step_increment']} "
+                f"({time.perf_counter() - t_i:.1f} s)")
     out = {
         "metric": "s/time-step + T_eff %HBM peak: FP32+learned vs FP64 APT, same 1e-8 residual",
         "value": round(s_per_step, 6),

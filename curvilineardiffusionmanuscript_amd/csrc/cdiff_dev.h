@@ -50,6 +50,8 @@ struct Ctrl {
   double last_l2;
   int valid_flags;  // step-start validity (bit 0: alpha domain, bits 1..: halo sides; bit 16+: source)
   int grow;         // inner correction loop: consecutive checks whose residual grew
+  int ref_iter;     // inner correction loop: iteration at which the residual last halved
+  double ref;       //   and the residual it halved to
 };
 
 // Julia `x * isfinite(x)` (Bool strong zero).

@@ -4,6 +4,10 @@
 
 namespace cdiff {
 
+// an inner correction solve whose residual has not halved within this many
+// sweeps stops (k_inner_finalize)
+constexpr int DC_STALL_SWEEPS = 192;
+
 struct DcCoefs {
   const double* c[9];
 };
@@ -13,9 +17,11 @@ __global__ void k_dc_update(Geo g, Ghost gh, KappaLaw kap, const double* rho, co
 __global__ void k_dc_residual(Geo g, int harmonic, double dt, const double* u, const double* a, const double* uprev,
                               DcCoefs cf, const double* S, const double* tzA, const double* tzB, float* r32,
                               double* psum, double* pmax);
-__global__ void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* psum, int nparts, double nfull, double eta,
-                                 int iter, int cap, double floor_frac, double rtol, double atol);
+__global__ void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* psum, const double* pmax, int nparts,
+                                 double nfull, int norm_kind, double eta, int iter, int cap, double floor_frac,
+                                 double rtol, double atol);
 __global__ void k_fill32(int64_t n, float v, float* dst);
+__global__ void k_ctrl_set_init(Ctrl* ctrl, const Ctrl* src, double rtol, double atol);
 __global__ void k_lin_metrics(Geo g, const double* g11, const double* g22, const double* c12, const double* ac,
                               const double* bc, float* m0, float* m1, float* m2, float* m3, float* m4);
 struct LinOut {

@@ -190,55 +190,103 @@ __global__ __launch_bounds__(256) void k_dc_residual(Geo g, int harmonic, double
 }
 
 // Inner-loop decision: stop the FP32 sweeps once the correction residual has
-// dropped below eta * (the outer FP64 residual), or after `cap` sweeps.
+// dropped below eta * (the outer FP64 residual), or after `cap` sweeps.  The
+// inner residual is measured in the outer residual's norm (norm_kind 0: RMS,
+// 1: max / sqrt(N)), so that the two are compared like for like.
 __global__ __launch_bounds__(1024) void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* __restrict__ psum,
-                                                         int nparts, double nfull, double eta, int iter, int cap,
+                                                         const double* __restrict__ pmax, int nparts, double nfull,
+                                                         int norm_kind, double eta, int iter, int cap,
                                                          double floor_frac, double rtol, double atol) {
   if (cin->done) return;
   constexpr int NT = 1024, U = 8;
-  __shared__ double ss[NT];
-  double as[U];
+  __shared__ double ss[NT], sm[NT];
+  double as[U], am[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) as[u] = 0.0;
+  for (int u = 0; u < U; ++u) as[u] = am[u] = 0.0;
   for (int base = 0; base < nparts; base += NT * U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int p = base + u * NT + threadIdx.x;
-      if (p < nparts) as[u] += psum[p];
+      if (p < nparts) {
+        as[u] += psum[p];
+        am[u] = nanmax(am[u], pmax[p]);
+      }
     }
   }
-  double s = 0.0;
+  double s = 0.0, m = 0.0;
 #pragma unroll
-  for (int u = 0; u < U; ++u) s += as[u];
+  for (int u = 0; u < U; ++u) {
+    s += as[u];
+    m = nanmax(m, am[u]);
+  }
   ss[threadIdx.x] = s;
+  sm[threadIdx.x] = m;
   __syncthreads();
   for (int o = NT / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) ss[threadIdx.x] += ss[threadIdx.x + o];
+    if (threadIdx.x < o) {
+      ss[threadIdx.x] += ss[threadIdx.x + o];
+      sm[threadIdx.x] = nanmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    }
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
-  const double L2 = sqrt(ss[0] / nfull);
+  const double L2 = norm_kind == 0 ? sqrt(ss[0] / nfull) : sm[0] / sqrt(nfull);
   const double prev = cin->last_l2;
   cin->last_l2 = L2;
   cin->nchecks += 1;
-  const double best = fmin(cin->init, L2);  // init: +inf after k_ctrl_reset
+  const double prev_best = cin->init;  // +inf after k_ctrl_reset
+  const double best = fmin(prev_best, L2);
   cin->init = best;
   cin->grow = (cin->nchecks > 1 && L2 > prev) ? cin->grow + 1 : 0;
   // stop at eta times the outer residual, or once the outer tolerance is in
   // reach: a fraction of max(atol, rtol * initial residual)
   double target = eta * cout->last_l2;
   if (floor_frac > 0.0) target = fmax(target, floor_frac * fmax(atol, rtol * cout->init));
-  // the sweeps are unstable for these pseudo-time parameters: growth far above
-  // the starting residual (that of e = 0, the outer residual), or, past the
-  // first 24 sweeps, growth at 3 checks in a row to 1.5x the smallest residual
-  // so far (a slow instability: measured on a 4096^2 blob problem, the
-  // residual fell to 6.7e-5 and then grew 1.1x per 12 sweeps, reaching the
-  // 100x bound only after ~1400 sweeps)
-  const bool diverged = L2 > 1e2 * cout->last_l2 || (iter >= 24 && cin->grow >= 3 && L2 > 1.5 * best);
-  if (!(L2 <= 1e300) || diverged || L2 <= target || iter >= cap) {
+  // the sweeps are unstable for these pseudo-time parameters: past the first
+  // 24 sweeps, a residual 100x above the smallest one so far, or growth at 3
+  // checks in a row to 1.5x it (a slow instability: measured on a 4096^2 blob
+  // problem, the residual fell to 6.7e-5 and then grew 1.1x per 12 sweeps).
+  // Not measured against the starting residual: the pseudo-transient's first
+  // sweeps overshoot it by up to ~1e3 (4096^2 Gaussian, kappa = 1: 355 ->
+  // 1.5e5 after 24 sweeps) and then fall
+  const bool diverged = iter >= 24 && (L2 > 1e2 * prev_best || (cin->grow >= 3 && L2 > 1.5 * best));
+  // the sweeps have stalled: the residual has not halved in the last
+  // DC_STALL_SWEEPS sweeps.  The FP32 rounding of the correction equation
+  // sets a floor under its residual (on the 4096^2 Gaussian with kappa = 1,
+  // 1.1e-3 of the outer residual, above eta); the correction reached so far
+  // goes to the outer FP64 loop, which decides convergence on its own residual
+  if (L2 <= 0.5 * cin->ref) {
+    cin->ref = L2;
+    cin->ref_iter = iter;
+  }
+  const bool stalled = iter - cin->ref_iter >= DC_STALL_SWEEPS;
+  if (!(L2 <= 1e300) || diverged || stalled || L2 <= target || iter >= cap) {
     cin->done = 1;
     cin->final_iter = iter;
     cin->status = !(L2 == L2) ? 1 : (diverged || !(L2 <= 1e300) ? 3 : 0);
+  }
+}
+
+// The denominator of the relative criterion from another control block's
+// first check (its init), with k_finalize's exit test re-evaluated: the
+// reference divides by the residual after its first pseudo-iteration
+// (PseudoTransientSolver.jl:280-293).
+__global__ void k_ctrl_set_init(Ctrl* ctrl, const Ctrl* src, double rtol, double atol) {
+  if (ctrl->done) return;
+  const double init = src->init;
+  const double ab = ctrl->abs_;
+  const double rel = ab / init;
+  ctrl->init = init;
+  ctrl->rel = rel;
+  int st = -1;
+  if (!isfinite(rel) || !isfinite(ab))
+    st = 1;  // CDIFF_NONFINITE
+  else if (rel <= rtol || ab <= atol)
+    st = 0;
+  if (st >= 0) {
+    ctrl->status = st;
+    ctrl->final_iter = 1;
+    ctrl->done = 1;
   }
 }
 

@@ -589,6 +589,27 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   }
   dc_dtau_scale = dc_dtau_scale_dflt;
   dc_theta_scale = dc_theta_scale_dflt;
+  // rtol > 0: the relative criterion divides by the reference's init_L2, the
+  // residual after the FP64 APT's first pseudo-iteration from T^n
+  // (PseudoTransientSolver.jl:280-293), whatever start the correction loop
+  // takes.  One FP64 sweep with its check, normed on the inner control block;
+  // the buffer it wrote gets T^n back.
+  const bool want_init = o.rtol > 0.0;
+  if (want_init) {
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl_in, 0.0, 0.0);
+    LAUNCH_CHECK("k_ctrl_reset");
+    if ((rc = launch_sweep64(o, dt, Vpdtau, cur, true, false))) return rc;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl_in, (const double*)part_sum,
+                       (const double*)part_max, nparts, 1, INT_MAX, 0.0, 0.0, N_full, o.norm_kind, 1);
+    LAUNCH_CHECK("k_finalize");
+    double* w = u[1 - cur];
+    hipLaunchKernelGGL(k_copy, grid_n(g.total), dim3(256), 0, stream, g.total, (const double*)Tdev, w);
+    LAUNCH_CHECK("k_copy");
+    if (dirichlet_outer) {
+      hipLaunchKernelGGL(k_halo_fill, grid_n((int64_t)g.n[0] * g.n[1] * g.n[2]), dim3(256), 0, stream, g, gh, w, 1);
+      LAUNCH_CHECK("k_halo_fill");
+    }
+  }
   // u[0] holds T (prologue); the first FP64 residual, of T with its BCs, into
   // u[1] -- when the learned warm start's feature pass has just evaluated it,
   // only its norm (the start with the smaller residual is evaluated next)
@@ -613,6 +634,12 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     if (dc_debug) std::fprintf(stderr, "[dc] warm start %s: residual %.3e\n", used ? "used" : "rejected", ctrl_h[2].abs_);
   }
   warm_pending = false;
+  if (want_init) {
+    hipLaunchKernelGGL(k_ctrl_set_init, dim3(1), dim3(1), 0, stream, ctrl, (const Ctrl*)ctrl_in, o.rtol, o.atol);
+    LAUNCH_CHECK("k_ctrl_set_init");
+    HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+  }
   int outer = 0;
   int64_t inner_total = 0;
   int inner_checks = 0;
@@ -643,7 +670,14 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl_in, 0.0, 0.0);
     LAUNCH_CHECK("inner reset");
     // (the coefficient-form sweep's coefficients come with the residual pass)
-    const int cap = fixed_inner > 0 ? fixed_inner : (int)std::min<int64_t>(INT_MAX - 1, std::max<int64_t>(1, total_cap - inner_total));
+    // the FP32 sweeps of a step never exceed max_iter: each correction solve
+    // gets what is left of it, and none is left once it is used up
+    const int64_t left = total_cap - inner_total;
+    if (left <= 0) {
+      dc_failed = true;
+      break;
+    }
+    const int cap = (int)std::min<int64_t>(fixed_inner > 0 ? fixed_inner : INT_MAX - 1, left);
     int it = 0, seg = 0;
     bool pending = false;
     int ebuf = 0;  // source buffer of the next inner launch
@@ -669,9 +703,9 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
         ends.push_back({tl, ebuf});
         if (chk_last) {
           hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(1024), 0, stream, ctrl_in, (const Ctrl*)ctrl,
-                             (const double*)(lin ? lin_psum : part_sum), lin ? lin_last_parts : nparts, N_full,
-                             fixed_inner > 0 ? 0.0 : eta, tl, cap, fixed_inner > 0 ? 0.0 : floor_frac, o.rtol,
-                             o.atol);
+                             (const double*)(lin ? lin_psum : part_sum), (const double*)(lin ? lin_pmax : part_max),
+                             lin ? lin_last_parts : nparts, N_full, o.norm_kind, fixed_inner > 0 ? 0.0 : eta, tl, cap,
+                             fixed_inner > 0 ? 0.0 : floor_frac, o.rtol, o.atol);
           LAUNCH_CHECK("k_inner_finalize");
         }
         t = tl + 1;
@@ -724,10 +758,6 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
                    ctrl_h[3].last_l2, ctrl_h[2].abs_);
     if (failed_now()) {
       dc_failed = true;
-      break;
-    }
-    if (inner_total > total_cap && !ctrl_h[2].done) {
-      ctrl_h[2].status = CDIFF_MAX_ITER;
       break;
     }
   }

@@ -198,6 +198,8 @@ __global__ void k_ctrl_reset(Ctrl* ctrl, double rtol, double atol) {
   ctrl->last_l2 = 0.0;
   ctrl->valid_flags = 0;
   ctrl->grow = 0;
+  ctrl->ref_iter = 0;
+  ctrl->ref = INFINITY;
 }
 
 __global__ void k_ctrl_validated(Ctrl* ctrl) {

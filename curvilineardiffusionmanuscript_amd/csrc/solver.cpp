@@ -44,6 +44,39 @@ void schedule(int last, int& m, int& k) {
   k = std::max(1, (last - m) / 2);
 }
 
+// A Float64 as Julia's show prints it (the text of the reference's error
+// messages, e.g. "$((_min, _max))", PseudoTransientSolver.jl:302-314,
+// validity_checks.jl:14-17): the shortest digits that read back as the same
+// double; fixed notation when the decimal point falls at -3..6 places
+// (0.0001, 123456.0), scientific otherwise (1.0e-5, 1.0e6); "Inf", "NaN",
+// and ".0" on integral values.
+std::string julia_float(double x) {
+  if (std::isnan(x)) return "NaN";
+  if (std::isinf(x)) return x < 0 ? "-Inf" : "Inf";
+  const std::string sign = std::signbit(x) ? "-" : "";
+  const double ax = std::fabs(x);
+  if (ax == 0.0) return sign + "0.0";
+  char e[40];
+  for (int p = 1; p <= 17; ++p) {
+    std::snprintf(e, sizeof e, "%.*e", p - 1, ax);
+    if (std::strtod(e, nullptr) == ax) break;
+  }
+  // ax = d1.d2..dn x 10^x10 = 0.d1..dn x 10^pt
+  std::string dig;
+  const char* c = e;
+  for (; *c && *c != 'e'; ++c)
+    if (*c >= '0' && *c <= '9') dig += *c;
+  while (dig.size() > 1 && dig.back() == '0') dig.pop_back();
+  const int n = (int)dig.size();
+  const int x10 = std::atoi(c + 1), pt = x10 + 1;
+  if (pt > -4 && pt <= 6) {
+    if (pt <= 0) return sign + "0." + std::string(-pt, '0') + dig;
+    if (pt < n) return sign + dig.substr(0, pt) + "." + dig.substr(pt);
+    return sign + dig + std::string(pt - n, '0') + ".0";
+  }
+  return sign + dig.substr(0, 1) + "." + (n > 1 ? dig.substr(1) : std::string("0")) + "e" + std::to_string(x10);
+}
+
 inline bool is_check(int it, int m, int k) { return (it >= m && it % k == 0) || it == 1; }
 
 int env_int(const char* name, int dflt) {
@@ -317,8 +350,8 @@ int cdiff_solver::validation_message(int status) {
   char buf[256];
   if (dom_bad) {
     if (any_nan) mn = mx = NAN;
-    std::snprintf(buf, sizeof buf, "Invalid %s in the domain extrema -> (%g, %g)", name, mn, mx);
-    err = buf;
+    err = std::string("Invalid ") + name + " in the domain extrema -> (" + julia_float(mn) + ", " + julia_float(mx) +
+          ")";
     return status;
   }
   int flo[3], fhi[3];
@@ -503,16 +536,13 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   if (status != CDIFF_OK) {
     if (stats) *stats = st;
     char buf[256];
-    if (status == CDIFF_NONFINITE) {
-      std::snprintf(buf, sizeof buf, "Non-finite error detected! abs_error = %g, rel_error = %g, exiting...", c.abs_,
-                    c.rel);
-      return fail(status, buf);
-    }
+    if (status == CDIFF_NONFINITE)
+      return fail(status, "Non-finite error detected! abs_error = " + julia_float(c.abs_) +
+                              ", rel_error = " + julia_float(c.rel) + ", exiting...");
     if (status == CDIFF_MAX_ITER) {
-      std::snprintf(buf, sizeof buf,
-                    "Maximum iteration limit reached (%d), abs_error = %g, rel_error = %g, exiting...", o.max_iter,
-                    c.abs_, c.rel);
-      return fail(status, buf);
+      std::snprintf(buf, sizeof buf, "Maximum iteration limit reached (%d), ", o.max_iter);
+      return fail(status, std::string(buf) + "abs_error = " + julia_float(c.abs_) + ", rel_error = " +
+                              julia_float(c.rel) + ", exiting...");
     }
     return validation_message(status);
   }

@@ -401,13 +401,24 @@ class PseudoTransientSolver:
             _check(lib.cdiff_set_kappa(self._h, C.byref(k), float(cp)), self._h)
             self._law, self._cp = law, float(cp)
         if rho is not None:
+            # the density on the device is replaced whenever the values passed
+            # differ from the last upload (compared by content: the caller may
+            # edit its array in place between steps)
             if np.isscalar(rho):
-                rho = np.full(self.shape, float(rho), order="F")
-            r = _fortran64(rho)
-            key = (id(rho), r.ctypes.data, float(r.sum()))
-            if key != self._rho_id:
-                _check(lib.cdiff_set_density(self._h, L.ptr(r)), self._h)
-                self._rho_id = key
+                key = float(rho)
+                if key != self._rho_id:
+                    if key == 1.0:
+                        _check(lib.cdiff_set_density(self._h, None), self._h)
+                    else:
+                        r = np.full(self.shape, key, order="F")
+                        _check(lib.cdiff_set_density(self._h, L.ptr(r)), self._h)
+                    self._rho_id = key
+            else:
+                r = _fortran64(rho)
+                last = self._rho_id
+                if not (isinstance(last, np.ndarray) and last.shape == r.shape and np.array_equal(last, r)):
+                    _check(lib.cdiff_set_density(self._h, L.ptr(r)), self._h)
+                    self._rho_id = r.copy(order="F")
 
     def step_resident(self, dt, rho=None, cp=1.0, kappa=1.0, **kwargs):
         """One step on the device-resident temperature; returns (stats, next_dt)."""
@@ -459,17 +470,46 @@ def update_conductivity(solver: PseudoTransientSolver, mesh, T, rho, cp, kappa):
     _check(L.load().cdiff_update_conductivity(solver._h, L.ptr(_fortran64(T))), solver._h)
 
 
+def _twilight_zone(solver: PseudoTransientSolver, mesh, rho, cp, t, tz, no_t):
+    """``tz=true`` of step! (PseudoTransientSolver.jl:189-191,276,281): the
+    update and the residual gain the forcing of the reference's manufactured
+    solution u = exp(-|x|^2/fwhm) exp(-damp t), fwhm = 1, damp = 4, centred at
+    the origin (twilight_zone.jl:101-106), at time ``t`` (``no_t``: without the
+    time factor).  The reference hard-codes its wavy test grid and kappa =
+    T^(5/2) (twilight_zone.jl:86-118,180-189); here the forcing is built for the
+    mesh and the conductivity law of the step.  ``tz=false`` removes a forcing
+    that an earlier ``tz=true`` step set (one set with ``set_forcing`` stays)."""
+    if tz:
+        from .twilight_zone import gaussian_forcing
+
+        r = 1.0 if rho is None else (float(rho) if np.isscalar(rho) else _fortran64(rho))
+        A, B = gaussian_forcing(
+            mesh, float(t), (0.0,) * solver.ndim, solver._law, fwhm=1.0, damp=4.0, no_t=bool(no_t), rho=r,
+            cp=float(cp), source=solver.source_term, axisymmetric=solver.axisymmetric,
+        )
+        solver.set_forcing(A, B)
+        solver._tz_forcing = True
+    elif getattr(solver, "_tz_forcing", False):
+        solver.set_forcing(None, None)
+        solver._tz_forcing = False
+
+
 def nonlinear_thermal_conduction_step(
     solver: PseudoTransientSolver, mesh, T, rho, cp, kappa, dt, *, t=0, tz=False, tz_dim=1, no_t=True,
     show_convergence=True, **kwargs
 ):
     """nonlinear_thermal_conduction_step!(::PseudoTransientSolver, ...) -> (stats, next_dt)
-    (src/nonlinear_thermal_conduction.jl:51-72).  ``T`` is overwritten in place."""
+    (src/nonlinear_thermal_conduction.jl:51-72).  ``T`` is overwritten in place.
+
+    ``t``, ``tz``, ``no_t``: the twilight-zone forcing of the reference's
+    manufactured solution (see ``_twilight_zone``).  ``tz_dim`` is accepted and
+    unused, as in the reference's wavy-grid forcing (twilight_zone.jl:54-194)."""
     if not isinstance(T, np.ndarray) or T.dtype != np.float64:
         raise TypeError("T must be a float64 numpy array of the full (halo-inclusive) mesh size")
     if T.shape != solver.shape:
         raise ValueError(f"T has shape {T.shape}, mesh needs {solver.shape}")
     solver._prepare(rho, cp, kappa)
+    _twilight_zone(solver, mesh, rho, cp, t, tz, no_t)
     o = _opts(**kwargs)
     st = L.Stats()
     nd = C.c_double(0.0)

@@ -383,6 +383,57 @@ double ora_next_dt(const ora_solver *s, const double *unew, const double *uold, 
   return dtn < g ? dtn : g;
 }
 
+/* A Float64 as Julia's show prints it (the text of the reference's error
+ * messages, e.g. "$((_min, _max))"): the shortest digits that read back as
+ * the same double; fixed notation when the decimal point falls at -3..6
+ * places (0.0001, 123456.0), scientific otherwise (1.0e-5, 1.0e6); "Inf",
+ * "NaN", and ".0" on integral values. */
+static void julia_float(double x, char *out, size_t cap) {
+  if (isnan(x)) {
+    snprintf(out, cap, "NaN");
+    return;
+  }
+  if (isinf(x)) {
+    snprintf(out, cap, x < 0 ? "-Inf" : "Inf");
+    return;
+  }
+  const char *sign = signbit(x) ? "-" : "";
+  const double ax = fabs(x);
+  if (ax == 0.0) {
+    snprintf(out, cap, "%s0.0", sign);
+    return;
+  }
+  char e[40];
+  for (int p = 1; p <= 17; p++) {
+    snprintf(e, sizeof e, "%.*e", p - 1, ax);
+    if (strtod(e, NULL) == ax) break;
+  }
+  /* ax = d1.d2..dn x 10^x10 = 0.d1..dn x 10^pt */
+  char dig[24], z[24];
+  int n = 0;
+  const char *c = e;
+  for (; *c && *c != 'e'; c++)
+    if (*c >= '0' && *c <= '9') dig[n++] = *c;
+  while (n > 1 && dig[n - 1] == '0') n--;
+  dig[n] = 0;
+  const int x10 = atoi(c + 1), pt = x10 + 1;
+  if (pt > -4 && pt <= 6) {
+    if (pt <= 0) {
+      memset(z, '0', (size_t)-pt);
+      z[-pt] = 0;
+      snprintf(out, cap, "%s0.%s%s", sign, z, dig);
+    } else if (pt < n) {
+      snprintf(out, cap, "%s%.*s.%s", sign, pt, dig, dig + pt);
+    } else {
+      memset(z, '0', (size_t)(pt - n));
+      z[pt - n] = 0;
+      snprintf(out, cap, "%s%s%s.0", sign, dig, z);
+    }
+  } else {
+    snprintf(out, cap, "%s%c.%se%d", sign, dig[0], n > 1 ? dig + 1 : "0", x10);
+  }
+}
+
 /* src/validity_checks.jl:1-42 over the domain plus the one-layer halo strips
  * of expand(domain, nhalo) (full extent of the other axes). Returns 0 when
  * valid; otherwise writes the reference's message into s->err. */
@@ -407,7 +458,10 @@ int ora_validate_scalar(ora_solver *s, const double *a, const char *name, int po
       }
   if (bad) {
     if (anynan) mn = mx = NAN;
-    snprintf(s->err, sizeof s->err, "Invalid %s in the domain extrema -> (%g, %g)", name, mn, mx);
+    char a0[64], a1[64];
+    julia_float(mn, a0, sizeof a0);
+    julia_float(mx, a1, sizeof a1);
+    snprintf(s->err, sizeof s->err, "Invalid %s in the domain extrema -> (%s, %s)", name, a0, a1);
     return ORA_INVALID_SCALAR;
   }
   int64_t flo[3], fhi[3];
@@ -552,17 +606,23 @@ int ora_step(ora_solver *s, double *T, double dt, const ora_opts *o, ora_stats *
       st->rel_err = rel;
       st->abs_err = absr;
       st->niter = iter;
+      char a0[64], a1[64];
+      julia_float(absr, a0, sizeof a0);
+      julia_float(rel, a1, sizeof a1);
       snprintf(s->err, sizeof s->err,
-               "Non-finite error detected! abs_error = %g, rel_error = %g, exiting...", absr, rel);
+               "Non-finite error detected! abs_error = %s, rel_error = %s, exiting...", a0, a1);
       return ORA_NONFINITE;
     }
     if (iter > o->max_iter) {
       st->rel_err = rel;
       st->abs_err = absr;
       st->niter = iter;
+      char a0[64], a1[64];
+      julia_float(absr, a0, sizeof a0);
+      julia_float(rel, a1, sizeof a1);
       snprintf(s->err, sizeof s->err,
-               "Maximum iteration limit reached (%d), abs_error = %g, rel_error = %g, exiting...",
-               o->max_iter, absr, rel);
+               "Maximum iteration limit reached (%d), abs_error = %s, rel_error = %s, exiting...",
+               o->max_iter, a0, a1);
       return ORA_MAX_ITER;
     }
     if (rel <= o->rtol || absr <= o->atol) break;

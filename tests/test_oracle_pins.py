@@ -414,3 +414,49 @@ def test_step_is_deterministic_across_thread_counts():
     O.set_num_threads(8)
     for T, it, nd in outs[1:]:
         assert np.array_equal(T, outs[0][0]) and it == outs[0][1] and nd == outs[0][2]
+
+
+def test_validate_scalar_messages_and_positivity():
+    # validity_checks.jl:1-42: the domain first (finite, and >= 0 with
+    # enforce_positivity), then the one-layer halo strips per axis, lo before
+    # hi; the domain message prints extrema((α_domain)) as a Julia tuple
+    # ("(-0.5, 1.0)"; Julia shows 1e-5 as 1.0e-5, 1e6 as 1.0e6, Inf, NaN)
+    mesh, _ = gaussian_2d(6, ni=5)
+    o = O.OracleSolver(mesh, [("neumann", 0.0)] * 4)
+    h = mesh.nhalo
+    base = np.ones(mesh.dims_full, order="F")
+
+    def msg(a, positivity):
+        with pytest.raises(O.OracleError) as e:
+            o.validate_scalar(a, "diffusivity", positivity)
+        return str(e.value)
+
+    a = base.copy(order="F")
+    a[h + 1, h + 2] = -0.5
+    assert msg(a, True) == "Invalid diffusivity in the domain extrema -> (-0.5, 1.0)"
+    o.validate_scalar(a, "diffusivity", False)  # negative values pass without positivity
+    a = base.copy(order="F")
+    a[h, h] = 1e-5
+    a[h + 2, h + 3] = np.inf
+    assert msg(a, False) == "Invalid diffusivity in the domain extrema -> (1.0e-5, Inf)"
+    a[h + 1, h + 1] = np.nan
+    assert msg(a, False) == "Invalid diffusivity in the domain extrema -> (NaN, NaN)"
+    a = base.copy(order="F")
+    a[h, h] = 2.5e6
+    a[h + 1, h] = 0.0001
+    a[h + 2, h] = -123456.0
+    assert msg(a, True) == "Invalid diffusivity in the domain extrema -> (-123456.0, 2.5e6)"
+    # halo strips: full extent of the other axes, corners included; lo first
+    a = base.copy(order="F")
+    a[-1, h + 1] = -1.0  # hi halo of axis 1
+    a[h + 2, 0] = -2.0  # lo halo of axis 2
+    o.validate_scalar(a, "diffusivity", False)
+    assert msg(a, True) == "Invalid diffusivity in the hi halo region for axis: 1"
+    a[0, 0] = np.nan  # corner: lo halo of axis 1 (and of axis 2)
+    assert msg(a, False) == "Invalid diffusivity in the lo halo region for axis: 1"
+    # layers beyond the first halo layer are not checked (expand(domain, 1))
+    mesh2, _ = gaussian_2d(6, ni=5, nhalo=2)
+    o2 = O.OracleSolver(mesh2, [("neumann", 0.0)] * 4)
+    a = np.ones(mesh2.dims_full, order="F")
+    a[0, 4] = np.nan
+    o2.validate_scalar(a, "diffusivity", True)
