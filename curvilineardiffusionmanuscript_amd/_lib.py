@@ -39,6 +39,7 @@ EXPORTS = (
     "cdiff_step_resident",
     "cdiff_apply_bcs",
     "cdiff_next_dt",
+    "cdiff_next_dt_device",
     "cdiff_get_field",
     "cdiff_set_field",
     "cdiff_get_last_iter",
@@ -141,6 +142,8 @@ def load(path: str = LIB_PATH):
     L.cdiff_step_resident.argtypes = [_vp, C.c_double, C.POINTER(Opts), C.POINTER(Stats), _dp]
     L.cdiff_apply_bcs.argtypes = [_vp, _dp]
     L.cdiff_next_dt.argtypes = [_vp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+    L.cdiff_next_dt_device.argtypes = [_vp, _vp, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, _vp,
+                                       _vp, C.c_char_p, C.c_size_t]
     L.cdiff_get_field.argtypes = [_vp, C.c_int, _dp]
     L.cdiff_set_field.argtypes = [_vp, C.c_int, _dp]
     L.cdiff_get_last_iter.argtypes = [_vp, C.POINTER(C.c_int32)]

@@ -41,6 +41,7 @@
 #ifndef CDIFF_H
 #define CDIFF_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -218,6 +219,22 @@ int cdiff_apply_bcs(cdiff_solver *h, double *u);
 /* next_dt(u_new, u_prev, dt; ...) on host arrays of the solver's full size. */
 int cdiff_next_dt(cdiff_solver *h, const double *u_new, const double *u_prev, double dt, double tol,
                   double maximum_u_change, double timestep_growth_factor, double *out);
+
+/* next_dt on device arrays, allocating nothing.  No counterpart in the
+ * reference: a variant of cdiff_next_dt for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight driver loops, harnesses
+ * that forbid hipMalloc).  u_new, u_prev: device arrays of the n elements of
+ * the full arrays (halo and corners included: next_dt takes its maxima over
+ * every element, timestep_control.jl:43-62); the next time step is written to
+ * the device double *out.  Enqueued on `stream` (a hipStream_t, NULL for the
+ * null stream) without waiting.  The reduction's partial maxima live in a
+ * fixed scratch array, one per GPU (256 KiB, held until the process ends),
+ * so n is at most 134,217,728 (CDIFF_BAD_ARG beyond), and calls that overlap
+ * on one GPU race on it.  On failure, errbuf (may be NULL) receives the
+ * message. */
+int cdiff_next_dt_device(const double *u_new, const double *u_prev, int64_t n, double dt, double tol,
+                         double maximum_u_change, double timestep_growth_factor, double *out, void *stream,
+                         char *errbuf, size_t errlen);
 
 /* Copy a solver field (cdiff_field) to a host array of the full size. */
 int cdiff_get_field(cdiff_solver *h, int field, double *out);
