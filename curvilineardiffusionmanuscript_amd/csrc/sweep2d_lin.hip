@@ -30,6 +30,7 @@
 // (sweep2d_lin_kernel), and wave-aligned region rows with tiles of 64 - 2K
 // columns (sweep2d_lin_rows_kernel, the default).
 #include "cdiff_dev.h"
+#include "lin2d_ops.h"
 #include "sweep_args.h"
 
 namespace cdiff {
@@ -130,14 +131,14 @@ __global__ __launch_bounds__(LNTHR) void sweep2d_lin_kernel(Lin2DArgs A) {
       const int lx = (code[p] >> 8) & 0xff, ly = (code[p] >> 16) & 0xff;
       if (lx >= s && ly >= s && ly <= RY - 1 - s) {
         if (lx <= RX - 2 - s) {
-          const float qp = px[p] * sE[t] - mx[p] * sE[t + 1];
-          qx[p] = qp + wx[p] * (qx[p] - qp);
+          float qp;
+          qx[p] = lin_face(px[p], mx[p], wx[p], sE[t], sE[t + 1], qx[p], qp);
           sX[t] = qx[p];
           if (CHECK && last) sPX[t] = qp;
         }
         if (lx <= RX - 1 - s && ly <= RY - 2 - s) {
-          const float qp = py[p] * sE[t] - my[p] * sE[t + RX];
-          qy[p] = qp + wy[p] * (qy[p] - qp);
+          float qp;
+          qy[p] = lin_face(py[p], my[p], wy[p], sE[t], sE[t + RX], qy[p], qp);
           sY[t] = qy[p];
           if (CHECK && last) sPY[t] = qp;
         }
@@ -150,16 +151,16 @@ __global__ __launch_bounds__(LNTHR) void sweep2d_lin_kernel(Lin2DArgs A) {
       const int t = tid + p * LNTHR;
       const int lx = (code[p] >> 8) & 0xff, ly = (code[p] >> 16) & 0xff;
       if ((code[p] & 1) && lx >= s + 1 && lx < RX - 1 - s && ly >= s + 1 && ly < RY - 1 - s) {
-        const float cross = ((sX[t + RX] + sX[t + RX - 1]) - (sX[t - RX] + sX[t - RX - 1])) +
-                            ((sY[t + 1] + sY[t - RX + 1]) - (sY[t - 1] + sY[t - RX - 1]));
-        const float D = m0[p] * sX[t] + m1[p] * sX[t - 1] + m2[p] * sY[t] + m3[p] * sY[t - RX] + m4[p] * cross;
-        const float en = e[p] + cc[p] * (rb[p] - D - e[p] * inv_dt);
+        const float cross = lin_cross(sX[t + RX], sX[t + RX - 1], sX[t - RX], sX[t - RX - 1], sY[t + 1],
+                                      sY[t - RX + 1], sY[t - 1], sY[t - RX - 1]);
+        const float D = lin_div(m0[p], m1[p], m2[p], m3[p], m4[p], sX[t], sX[t - 1], sY[t], sY[t - RX], cross);
+        const float en = lin_update(e[p], cc[p], rb[p], D, inv_dt);
         if (CHECK && last) {
-          const float crossp = ((sPX[t + RX] + sPX[t + RX - 1]) - (sPX[t - RX] + sPX[t - RX - 1])) +
-                               ((sPY[t + 1] + sPY[t - RX + 1]) - (sPY[t - 1] + sPY[t - RX - 1]));
+          const float crossp = lin_cross(sPX[t + RX], sPX[t + RX - 1], sPX[t - RX], sPX[t - RX - 1], sPY[t + 1],
+                                         sPY[t - RX + 1], sPY[t - 1], sPY[t - RX - 1]);
           const float Dp =
-              m0[p] * sPX[t] + m1[p] * sPX[t - 1] + m2[p] * sPY[t] + m3[p] * sPY[t - RX] + m4[p] * crossp;
-          const double res = (double)(rb[p] - en * inv_dt - Dp);
+              lin_div(m0[p], m1[p], m2[p], m3[p], m4[p], sPX[t], sPX[t - 1], sPY[t], sPY[t - RX], crossp);
+          const double res = (double)lin_residual(rb[p], en, inv_dt, Dp);
           sq += res * res;
           smax = nanmax(smax, fabs(res));
           if (res != res) smax = res;
@@ -302,14 +303,14 @@ __global__ __launch_bounds__(NW * WAVE) void sweep2d_lin_rows_kernel(Lin2DArgs A
       const int t = ly * RX + lx;
       if (ly >= s && ly <= RY - 1 - s) {
         if (fx_on) {
-          const float qp = px[p] * sE[t] - mx[p] * sE[t + 1];
-          qx[p] = qp + wx[p] * (qx[p] - qp);
+          float qp;
+          qx[p] = lin_face(px[p], mx[p], wx[p], sE[t], sE[t + 1], qx[p], qp);
           sX[t] = qx[p];
           if (CHECK && last) sPX[t] = qp;
         }
         if (fy_on && ly <= RY - 2 - s) {
-          const float qp = py[p] * sE[t] - my[p] * sE[t + RX];
-          qy[p] = qp + wy[p] * (qy[p] - qp);
+          float qp;
+          qy[p] = lin_face(py[p], my[p], wy[p], sE[t], sE[t + RX], qy[p], qp);
           sY[t] = qy[p];
           if (CHECK && last) sPY[t] = qp;
         }
@@ -321,16 +322,16 @@ __global__ __launch_bounds__(NW * WAVE) void sweep2d_lin_rows_kernel(Lin2DArgs A
       const int ly = w + NW * p;
       const int t = ly * RX + lx;
       if (ly >= s + 1 && ly < RY - 1 - s && c_on && ((dom >> p) & 1u)) {
-        const float cross = ((sX[t + RX] + sX[t + RX - 1]) - (sX[t - RX] + sX[t - RX - 1])) +
-                            ((sY[t + 1] + sY[t - RX + 1]) - (sY[t - 1] + sY[t - RX - 1]));
-        const float D = m0[p] * sX[t] + m1[p] * sX[t - 1] + m2[p] * sY[t] + m3[p] * sY[t - RX] + m4[p] * cross;
-        const float en = e[p] + cc[p] * (rb[p] - D - e[p] * inv_dt);
+        const float cross = lin_cross(sX[t + RX], sX[t + RX - 1], sX[t - RX], sX[t - RX - 1], sY[t + 1],
+                                      sY[t - RX + 1], sY[t - 1], sY[t - RX - 1]);
+        const float D = lin_div(m0[p], m1[p], m2[p], m3[p], m4[p], sX[t], sX[t - 1], sY[t], sY[t - RX], cross);
+        const float en = lin_update(e[p], cc[p], rb[p], D, inv_dt);
         if (CHECK && last) {
-          const float crossp = ((sPX[t + RX] + sPX[t + RX - 1]) - (sPX[t - RX] + sPX[t - RX - 1])) +
-                               ((sPY[t + 1] + sPY[t - RX + 1]) - (sPY[t - 1] + sPY[t - RX - 1]));
+          const float crossp = lin_cross(sPX[t + RX], sPX[t + RX - 1], sPX[t - RX], sPX[t - RX - 1], sPY[t + 1],
+                                         sPY[t - RX + 1], sPY[t - 1], sPY[t - RX - 1]);
           const float Dp =
-              m0[p] * sPX[t] + m1[p] * sPX[t - 1] + m2[p] * sPY[t] + m3[p] * sPY[t - RX] + m4[p] * crossp;
-          const double res = (double)(rb[p] - en * inv_dt - Dp);
+              lin_div(m0[p], m1[p], m2[p], m3[p], m4[p], sPX[t], sPX[t - 1], sPY[t], sPY[t - RX], crossp);
+          const double res = (double)lin_residual(rb[p], en, inv_dt, Dp);
           sq += res * res;
           smax = nanmax(smax, fabs(res));
           if (res != res) smax = res;

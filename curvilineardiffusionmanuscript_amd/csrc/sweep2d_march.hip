@@ -1,6 +1,6 @@
 // sweep2d_march.hip -- the FP32 correction sweep of sweep2d_lin.hip (same
-// coefficient form, same arithmetic per face and per cell), K iterations per
-// launch as a pipeline that marches up the rows.
+// coefficient form, same arithmetic per face and per cell: lin2d_ops.h), K
+// iterations per launch as a pipeline that marches up the rows.
 //
 // One wavefront owns a strip of 64 - 2K columns (lane l holds column
 // x0 - K + l: the strip plus a K-cell ring on each side) and a chunk of rows
@@ -12,17 +12,41 @@
 // from the iteration n - 1 values of the rows around it (kept in registers,
 // east and west neighbours by DPP wavefront shifts).  Stage n needs iteration n - 1
 // of row t - n + 1, which stage n - 1 produced in the same step, so the
-// stages run in order.  Each row's coefficients stay in registers for the K
-// steps that use them.  Against the tiled sweep, the ring is re-read only
+// stages run in order.  Against the tiled sweep, the ring is re-read only
 // along x (64 / (64 - 2K)) and 2K rows per chunk, and there are no barriers;
 // the neighbouring strips of a chunk run on one XCD and share the cache lines
 // at their edges in its L2.
+//
+// Registers.  A row's coefficients are used by K consecutive steps, so they
+// sit in rings of K + 1 row slots (x faces of rows t + 1 .. t - K + 1, y faces
+// and cells of rows t .. t - K): the row loop is unrolled K + 1 times, so every
+// slot index is a compile-time constant and no coefficient is ever moved.
+// The x faces and the correction of row t + 1 are fetched during step t (stage
+// 1 needs them first), the y faces and cells of row t during step t (first
+// needed at step t + 1).
+//
+// Loads.  Every coefficient array (P, M, W of both face sets, C, the right-hand
+// side, the folded metrics) holds 0 outside its valid range, so a lane reads
+// without a mask: from its own cell when that lies in [lo - 1, hi + 1] on
+// both axes, else from column (or row) hi + 1, where every array read here
+// is 0.  The correction and its fluxes are masked after the load (their
+// halo entries are not kept at 0 by every writer).
+//
+// Bound.  HBM: at 4096^2 (six iterations per launch, 53 us per iteration) a
+// copy without the arithmetic takes as long, one without the loads 27 us.  Its
+// access pattern -- each wavefront reads 256 bytes of 16 arrays per row, the
+// strips of a chunk in step, 40 chunks apart -- moves ~4.1 TB/s in isolation
+// (a grid-stride read of the same arrays, 6.2 TB/s); neither more rows in
+// flight per wavefront nor interleaved coefficient records change that much.
 //
 // Values outside the dependency cone of the strip's cells (the ring's edge
 // lanes and the first rows of a chunk) are computed from incomplete
 // neighbours and never reach a written cell: after stage n, lanes
 // n .. 63 - n and rows from ya - K + n up are exact.
+#include <type_traits>
+
 #include "cdiff_dev.h"
+#include "lin2d_ops.h"
 #include "sweep_args.h"
 
 namespace cdiff {
@@ -30,46 +54,6 @@ namespace cdiff {
 namespace {
 
 constexpr int MWAVES = 4;  // independent wavefronts per block
-
-// what one lane loads per row
-struct RowIn {
-  float e, qx, qy;
-  float px, mx, wx;                       // x face (lane's column, this row)
-  float py, my, wy;                       // y face (between this row and the next)
-  float rb, cc, m0, m1, m2, m3, m4;       // cell
-};
-
-__device__ __forceinline__ void load_row(const Lin2DArgs& A, int i, int r, bool icol, bool xcol, RowIn& v) {
-  const Geo& g = A.g;
-  v = RowIn{};
-  const bool rin = r >= g.lo[1] && r <= g.hi[1];
-  const bool d = icol && rin;
-  const bool xon = xcol && rin;
-  const bool yon = icol && r >= g.lo[1] - 1 && r <= g.hi[1];
-  const int64_t c = at(g, i, r, 0);
-  if (d) {
-    if (!A.zero_src) v.e = A.e_src[c];
-    v.rb = A.base[c];
-    v.cc = A.cc[c];
-    v.m0 = A.m[0][c];
-    v.m1 = A.m[1][c];
-    v.m2 = A.m[2][c];
-    v.m3 = A.m[3][c];
-    v.m4 = A.m[4][c];
-  }
-  if (xon) {
-    if (!A.zero_src) v.qx = A.qx_src[c];
-    v.px = A.fx[0][c];
-    v.mx = A.fx[1][c];
-    v.wx = A.fx[2][c];
-  }
-  if (yon) {
-    if (!A.zero_src) v.qy = A.qy_src[c];
-    v.py = A.fy[0][c];
-    v.my = A.fy[1][c];
-    v.wy = A.fy[2][c];
-  }
-}
 
 // east / west neighbour by a DPP wavefront shift by one lane (the edge lane
 // reads 0; edge lanes lie outside every written cell's dependency cone)
@@ -80,11 +64,35 @@ __device__ __forceinline__ float dpp_from_right(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));  // wave_shl:1
 }
 
+// a float at a byte offset from a (wave-uniform) array base: a 32-bit offset
+// register against the base in scalar registers
+__device__ __forceinline__ float ldf(const float* __restrict__ base, uint32_t byte_offset) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_offset);
+}
+
+__device__ __forceinline__ void stf(float* __restrict__ base, uint32_t byte_offset, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset) = v;
+}
+
+// the phases 0 .. R - 1 of the unrolled row loop; false once the last row is done
+template <int P, int R, class F>
+__device__ __forceinline__ bool march_phases(F& step, int& t, int t1) {
+  if constexpr (P == R) {
+    return true;
+  } else {
+    step(std::integral_constant<int, P>{}, t);
+    if (++t > t1) return false;
+    return march_phases<P + 1, R>(step, t, t1);
+  }
+}
+
 }  // namespace
 
-template <int K, bool CHECK>
-__global__ __launch_bounds__(MWAVES * WAVE) void sweep2d_lin_march_kernel(Lin2DArgs A) {
+template <int K, bool CHECK, int PF>
+__global__ __launch_bounds__(MWAVES * WAVE) __attribute__((amdgpu_waves_per_eu(PF == 1 && K <= 6 ? 3 : 2))) void
+sweep2d_lin_march_kernel(Lin2DArgs A) {
   constexpr int TX = WAVE - 2 * K;
+  constexpr int R = K + PF;  // row slots of the coefficient rings (PF rows fetched ahead)
   if (A.ctrl->done) return;
   const Geo& g = A.g;
   const int lane = threadIdx.x & (WAVE - 1);
@@ -105,41 +113,89 @@ __global__ __launch_bounds__(MWAVES * WAVE) void sweep2d_lin_march_kernel(Lin2DA
     }
     return;
   }
-  const int x0 = g.lo[0] + strip * TX;
+  const int ilo = g.lo[0], ihi = g.hi[0], jlo = g.lo[1], jhi = g.hi[1];
+  const int x0 = ilo + strip * TX;
   const int i = x0 - K + lane;
-  const int ya = g.lo[1] + chunk * A.rows_per_chunk;
-  const int yb = min(ya + A.rows_per_chunk, g.hi[1] + 1);
-  const bool icol = i >= g.lo[0] && i <= g.hi[0];
-  const bool xcol = i >= g.lo[0] - 1 && i <= g.hi[0];
+  const int ya = jlo + chunk * A.rows_per_chunk;
+  const int yb = min(ya + A.rows_per_chunk, jhi + 1);
+  const bool icol = i >= ilo && i <= ihi;
+  const bool xcol = i >= ilo - 1 && i <= ihi;
   const bool cols = lane >= K && lane < K + TX;
+  const int il = (i >= ilo - 1 && i <= ihi + 1) ? i : ihi + 1;  // the column this lane reads
   const float inv_dt = A.inv_dt;
+  const bool zero_src = A.zero_src != 0;
+  // byte offset of (il, r) in every FP32 array: rows outside [lo - 1, hi + 1]
+  // read row hi + 1
+  auto offset = [&](int r) -> uint32_t {
+    const int rl = (r >= jlo - 1 && r <= jhi + 1) ? r : jhi + 1;
+    return (uint32_t)(g.off + il + g.pitch * (int64_t)rl) * 4u;
+  };
 
+  // coefficient rings: FX[s] the x faces (P, M, W) of the row in slot s and
+  // its correction and fluxes (e, q_x, q_y; used when the row enters), CY[s]
+  // its y faces (P, M, W), right-hand side, C and folded metrics m0..m4; row r
+  // sits in slot (r - t0) mod R
+  float FX[R][6], CY[R][10];
   // E[n][0]: iteration-n correction of row t - n, E[n][1]: of row t - n - 1
   // QX[n][0..2]: iteration-n x faces of rows t - n + 1, t - n, t - n - 1
   // QY[n][0..1]: iteration-n y faces of rows t - n, t - n - 1
   // (level 0: the loaded values, rows t and t - 1)
   float E[K + 1][2], QX[K + 1][3], QY[K + 1][2];
-  // coefficients: FX[n] x faces of row t - n (n < K); CY[n] y faces and cell of row t - n (n <= K)
-  float FX[K][3], CY[K + 1][10];
   float QPX[3], QPY[2];  // iteration-K q' (check launches)
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) FX[s][c] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 10; ++c) CY[s][c] = 0.f;
+  }
 #pragma unroll
   for (int n = 0; n <= K; ++n) {
     E[n][0] = E[n][1] = 0.f;
     QX[n][0] = QX[n][1] = QX[n][2] = 0.f;
     QY[n][0] = QY[n][1] = 0.f;
-#pragma unroll
-    for (int c = 0; c < 10; ++c) CY[n][c] = 0.f;
   }
-#pragma unroll
-  for (int n = 0; n < K; ++n) FX[n][0] = FX[n][1] = FX[n][2] = 0.f;
   QPX[0] = QPX[1] = QPX[2] = QPY[0] = QPY[1] = 0.f;
   double sq = 0.0, smax = 0.0;
 
+  // x faces and the correction of row r (the correction masked to its valid
+  // entries; on the first launch of a solve it is 0)
+  auto load_x = [&](int r, float* fx) {
+    const uint32_t o = offset(r);
+    fx[0] = ldf(A.fx[0], o);
+    fx[1] = ldf(A.fx[1], o);
+    fx[2] = ldf(A.fx[2], o);
+    const float e = ldf(A.e_src, o), qx = ldf(A.qx_src, o), qy = ldf(A.qy_src, o);
+    const bool rin = !zero_src && r >= jlo && r <= jhi;
+    fx[3] = (icol && rin) ? e : 0.f;
+    fx[4] = (xcol && rin) ? qx : 0.f;
+    fx[5] = (icol && !zero_src && r >= jlo - 1 && r <= jhi) ? qy : 0.f;
+  };
+  // y faces and cell coefficients of row r
+  auto load_y = [&](int r, float* cy) {
+    const uint32_t o = offset(r);
+    cy[0] = ldf(A.fy[0], o);
+    cy[1] = ldf(A.fy[1], o);
+    cy[2] = ldf(A.fy[2], o);
+    cy[3] = ldf(A.base, o);
+    cy[4] = ldf(A.cc, o);
+    cy[5] = ldf(A.m[0], o);
+    cy[6] = ldf(A.m[1], o);
+    cy[7] = ldf(A.m[2], o);
+    cy[8] = ldf(A.m[3], o);
+    cy[9] = ldf(A.m[4], o);
+  };
+
   const int t0 = ya - K, t1 = yb - 1 + K;
-  RowIn nxt;
-  load_row(A, i, t0, icol, xcol, nxt);
-  for (int t = t0; t <= t1; ++t) {
-    // ---- shift the histories by one row
+#pragma unroll
+  for (int d = 0; d < PF; ++d) {
+    if (t0 + d <= t1) load_x(t0 + d, FX[d]);
+    if (d >= 1 && t0 + d - 1 < t1) load_y(t0 + d - 1, CY[d - 1]);
+  }
+
+  auto step = [&](auto Pc, int t) {
+    constexpr int P = decltype(Pc)::value;
+    // ---- shift the correction and flux histories by one row
 #pragma unroll
     for (int n = K; n >= 0; --n) {
       E[n][1] = E[n][0];
@@ -147,82 +203,62 @@ __global__ __launch_bounds__(MWAVES * WAVE) void sweep2d_lin_march_kernel(Lin2DA
       QX[n][1] = QX[n][0];
       QY[n][1] = QY[n][0];
     }
-#pragma unroll
-    for (int n = K; n >= 1; --n) {
-#pragma unroll
-      for (int c = 0; c < 10; ++c) CY[n][c] = CY[n - 1][c];
-    }
-#pragma unroll
-    for (int n = K - 1; n >= 1; --n) {
-      FX[n][0] = FX[n - 1][0];
-      FX[n][1] = FX[n - 1][1];
-      FX[n][2] = FX[n - 1][2];
-    }
     QPX[2] = QPX[1];
     QPX[1] = QPX[0];
     QPY[1] = QPY[0];
-    // ---- row t enters at level 0; row t + 1 is fetched while it is used
-    E[0][0] = nxt.e;
-    QX[0][0] = nxt.qx;
-    QY[0][0] = nxt.qy;
-    FX[0][0] = nxt.px;
-    FX[0][1] = nxt.mx;
-    FX[0][2] = nxt.wx;
-    CY[0][0] = nxt.py;
-    CY[0][1] = nxt.my;
-    CY[0][2] = nxt.wy;
-    CY[0][3] = nxt.rb;
-    CY[0][4] = nxt.cc;
-    CY[0][5] = nxt.m0;
-    CY[0][6] = nxt.m1;
-    CY[0][7] = nxt.m2;
-    CY[0][8] = nxt.m3;
-    CY[0][9] = nxt.m4;
-    if (t < t1) load_row(A, i, t + 1, icol, xcol, nxt);
+    // ---- row t enters at level 0; the loads of the next rows are in flight
+    //      while the stages run
+    E[0][0] = FX[P][3];
+    QX[0][0] = FX[P][4];
+    QY[0][0] = FX[P][5];
+    if (t + PF - 1 < t1) load_y(t + PF - 1, CY[(P + PF - 1) % R]);
+    if (t + PF <= t1) load_x(t + PF, FX[(P + PF) % R]);
 
     // ---- stages
 #pragma unroll
     for (int n = 1; n <= K; ++n) {
       const int j = t - n;  // row of this stage's cells and y faces
+      const float* fx = FX[(P - n + 1 + 2 * R) % R];  // x faces of row j + 1
+      const float* cy = CY[(P - n + 2 * R) % R];      // y faces and cells of row j
       // x faces of row j + 1
       {
         const float e0 = E[n - 1][0];
         const float e1 = dpp_from_right(e0);
         const float qold = n == 1 ? QX[0][0] : QX[n - 1][1];
-        const float qp = FX[n - 1][0] * e0 - FX[n - 1][1] * e1;
-        QX[n][0] = qp + FX[n - 1][2] * (qold - qp);
+        float qp;
+        QX[n][0] = lin_face(fx[0], fx[1], fx[2], e0, e1, qold, qp);
         if (CHECK && n == K) QPX[0] = qp;
       }
       // y faces of row j
       {
-        const float e0 = E[n - 1][1], e1 = E[n - 1][0];
-        const float qold = QY[n - 1][1];
-        const float qp = CY[n][0] * e0 - CY[n][1] * e1;
-        QY[n][0] = qp + CY[n][2] * (qold - qp);
+        float qp;
+        QY[n][0] = lin_face(cy[0], cy[1], cy[2], E[n - 1][1], E[n - 1][0], QY[n - 1][1], qp);
         if (CHECK && n == K) QPY[0] = qp;
       }
       // cells of row j
       {
         const float xc = QX[n][1], xn = QX[n][0], xs = QX[n][2];
         const float yc = QY[n][0], ys = QY[n][1];
-        const float xw = dpp_from_left(xc), xnw = dpp_from_left(xn), xsw = dpp_from_left(xs);
-        const float ye = dpp_from_right(yc), yes = dpp_from_right(ys), yw = dpp_from_left(yc), yws = dpp_from_left(ys);
-        const float cross = ((xn + xnw) - (xs + xsw)) + ((ye + yes) - (yw + yws));
-        const float D = CY[n][5] * xc + CY[n][6] * xw + CY[n][7] * yc + CY[n][8] * ys + CY[n][9] * cross;
+        const float cross = lin_cross(xn, dpp_from_left(xn), xs, dpp_from_left(xs), dpp_from_right(yc),
+                                      dpp_from_right(ys), dpp_from_left(yc), dpp_from_left(ys));
+        const float D = lin_div(cy[5], cy[6], cy[7], cy[8], cy[9], xc, dpp_from_left(xc), yc, ys, cross);
         const float eo = E[n - 1][1];
-        const float en = eo + CY[n][4] * (CY[n][3] - D - eo * inv_dt);
-        const bool dom = icol && j >= g.lo[1] && j <= g.hi[1];
-        E[n][0] = dom ? en : eo;
+        const float en = lin_update(eo, cy[4], cy[3], D, inv_dt);
+        // no mask: outside the domain the correction was loaded as +0 and C,
+        // the right-hand side and the metric terms are 0, so the update
+        // returns +0 again as long as the fluxes are finite (a solve whose
+        // fluxes are not is dropped as diverged)
+        E[n][0] = en;
+        const bool dom = icol && (unsigned)(j - jlo) <= (unsigned)(jhi - jlo);
         if (CHECK && n == K) {
           // the residual of iteration K with its q' (all lanes shuffle; only
           // the strip's domain cells of the chunk's rows are summed)
           const float pxc = QPX[1], pxn = QPX[0], pxs = QPX[2];
           const float pyc = QPY[0], pys = QPY[1];
-          const float pxw = dpp_from_left(pxc), pxnw = dpp_from_left(pxn), pxsw = dpp_from_left(pxs);
-          const float pye = dpp_from_right(pyc), pyes = dpp_from_right(pys), pyw = dpp_from_left(pyc), pyws = dpp_from_left(pys);
-          const float crossp = ((pxn + pxnw) - (pxs + pxsw)) + ((pye + pyes) - (pyw + pyws));
-          const float Dp = CY[n][5] * pxc + CY[n][6] * pxw + CY[n][7] * pyc + CY[n][8] * pys + CY[n][9] * crossp;
-          const double res = (dom && cols && j >= ya && j < yb) ? (double)(CY[n][3] - en * inv_dt - Dp) : 0.0;
+          const float crossp = lin_cross(pxn, dpp_from_left(pxn), pxs, dpp_from_left(pxs), dpp_from_right(pyc),
+                                         dpp_from_right(pys), dpp_from_left(pyc), dpp_from_left(pys));
+          const float Dp = lin_div(cy[5], cy[6], cy[7], cy[8], cy[9], pxc, dpp_from_left(pxc), pyc, pys, crossp);
+          const double res = (dom && cols && j >= ya && j < yb) ? (double)lin_residual(cy[3], en, inv_dt, Dp) : 0.0;
           sq += res * res;
           smax = nanmax(smax, fabs(res));
           if (res != res) smax = res;
@@ -231,14 +267,21 @@ __global__ __launch_bounds__(MWAVES * WAVE) void sweep2d_lin_march_kernel(Lin2DA
     }
 
     // ---- writes: iteration K of cells and y faces of row t - K, x faces of row t - K + 1
+    //      (32-bit byte offsets from the lane's own column; only lanes of
+    //      valid cells and faces store)
     const int jc = t - K, jx = t - K + 1;
+    const uint32_t oc = (uint32_t)(g.off + i + g.pitch * (int64_t)jc) * 4u;
     if (jc >= ya && jc < yb && cols && icol) {
-      const int64_t c = at(g, i, jc, 0);
-      A.e_dst[c] = E[K][0];
-      A.qy_dst[c] = QY[K][0];
+      stf(A.e_dst, oc, E[K][0]);
+      stf(A.qy_dst, oc, QY[K][0]);
     }
-    if (jc == g.lo[1] - 1 && cols && i <= g.hi[0]) A.qy_dst[at(g, i, jc, 0)] = QY[K][0];
-    if (jx >= ya && jx < yb && ((cols && icol) || i == g.lo[0] - 1)) A.qx_dst[at(g, i, jx, 0)] = QX[K][0];
+    if (jc == jlo - 1 && cols && i <= ihi) stf(A.qy_dst, oc, QY[K][0]);
+    if (jx >= ya && jx < yb && ((cols && icol) || i == ilo - 1))
+      stf(A.qx_dst, oc + (uint32_t)g.pitch * 4u, QX[K][0]);
+  };
+
+  int t = t0;
+  while (march_phases<0, R>(step, t, t1)) {
   }
   if (CHECK) {
     sq = wave_sum(sq);
@@ -256,28 +299,35 @@ int lin2d_march_blocks(int strips, int chunks) {
 
 int lin2d_march_parts(int strips, int chunks) { return MWAVES * lin2d_march_blocks(strips, chunks); }
 
-void launch_sweep2d_march(const Lin2DArgs& A, int kmax, bool check, hipStream_t s) {
-  const int nb = lin2d_march_blocks(A.strips, A.chunks);
+template <int PF>
+void launch_march_pf(const Lin2DArgs& A, int kmax, bool check, hipStream_t s, int nb) {
   switch (kmax) {
     case 4:
       if (check)
-        hipLaunchKernelGGL((sweep2d_lin_march_kernel<4, true>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
+        hipLaunchKernelGGL((sweep2d_lin_march_kernel<4, true, PF>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
       else
-        hipLaunchKernelGGL((sweep2d_lin_march_kernel<4, false>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
+        hipLaunchKernelGGL((sweep2d_lin_march_kernel<4, false, PF>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
       break;
     case 8:
       if (check)
-        hipLaunchKernelGGL((sweep2d_lin_march_kernel<8, true>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
+        hipLaunchKernelGGL((sweep2d_lin_march_kernel<8, true, PF>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
       else
-        hipLaunchKernelGGL((sweep2d_lin_march_kernel<8, false>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
+        hipLaunchKernelGGL((sweep2d_lin_march_kernel<8, false, PF>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
       break;
     default:
       if (check)
-        hipLaunchKernelGGL((sweep2d_lin_march_kernel<6, true>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
+        hipLaunchKernelGGL((sweep2d_lin_march_kernel<6, true, PF>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
       else
-        hipLaunchKernelGGL((sweep2d_lin_march_kernel<6, false>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
+        hipLaunchKernelGGL((sweep2d_lin_march_kernel<6, false, PF>), dim3(nb), dim3(MWAVES * WAVE), 0, s, A);
       break;
   }
+}
+
+void launch_sweep2d_march(const Lin2DArgs& A, int kmax, bool check, hipStream_t s) {
+  // rows fetched ahead: 1 (4096^2, six iterations per launch: 53 us per
+  // iteration; 2 and 3 rows ahead need 182 and 199 VGPRs, two wavefronts per
+  // SIMD, and take 60 us)
+  launch_march_pf<1>(A, kmax, check, s, lin2d_march_blocks(A.strips, A.chunks));
 }
 
 }  // namespace cdiff
