@@ -214,6 +214,10 @@ int cdiff_create(int ndim, const int64_t* dims_full, int nhalo, int mean, const 
   g.plane = g.pitch * g.n[1];
   g.off = ALIGN - nhalo;
   g.total = round_up(g.off + g.plane * g.n[2], ALIGN) + ALIGN;
+  if (g.total >= ((int64_t)1 << 31)) {
+    *out = h;
+    return h->fail(CDIFF_BAD_ARG, "mesh too large: a field would hold 2^31 or more elements");
+  }
   h->N_full = (double)g.n[0] * g.n[1] * g.n[2];
   set_law(h, kappa, cp);
   int rc = build_ghost(h);
@@ -243,7 +247,7 @@ int cdiff_create(int ndim, const int64_t* dims_full, int nhalo, int mean, const 
   HIP_TRY(hipMalloc((void**)&h->ctrl, sizeof(Ctrl)));
   h->owned.push_back(h->ctrl);
   HIP_TRY(hipHostMalloc((void**)&h->ctrl_h, 4 * sizeof(Ctrl), hipHostMallocDefault));
-  HIP_TRY(hipMalloc((void**)&h->red, sizeof(double) * 2048));
+  HIP_TRY(hipMalloc((void**)&h->red, sizeof(double) * (8 + STEP_BLOCKS)));
   h->owned.push_back(h->red);
   HIP_TRY(hipHostMalloc((void**)&h->red_h, sizeof(double) * 8, hipHostMallocDefault));
   for (int e = 0; e < 2; ++e) HIP_TRY(hipEventCreateWithFlags(&h->ev[e], hipEventDisableTiming));

@@ -95,7 +95,9 @@ __device__ __forceinline__ T half_pow(T x, int n) {
 }
 
 // alpha = |kappa(rho, T) / (rho cp)| (src/pseudo_transient/conductivity.jl:7-10).
-template <typename T>
+// UNIT: no density array and cp = 1, so rho cp = 1 and the division, which
+// would return kappa exactly, is left out.
+template <typename T, bool UNIT = false>
 __device__ __forceinline__ T alpha_of(const KappaLaw& k, T rho, T u) {
   T kap;
   if (!isfinite(u)) {
@@ -109,6 +111,7 @@ __device__ __forceinline__ T alpha_of(const KappaLaw& k, T rho, T u) {
       kap = (T)k.kappa0 * rt * tt;
     }
   }
+  if (UNIT) return fabs(kap);
   return fabs(kap / (rho * (T)k.cp));
 }
 

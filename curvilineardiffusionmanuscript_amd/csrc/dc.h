@@ -50,7 +50,6 @@ struct DcOuter2D {
   const double* u_old;
   const float* e;  // nullable: no correction
   double* u_new;
-  double* a64;
   const double* uprev;
   const double* c[5];  // g11 g22 c12 ac bc
   const double *S, *tzA, *tzB;
@@ -61,7 +60,8 @@ struct DcOuter2D {
   double* psum;
   double* pmax;
 };
-__global__ void k_dc_outer2d(DcOuter2D A);
+// unit: no density array and cp = 1
+void launch_dc_outer2d(const DcOuter2D& A, int nblk, bool unit, hipStream_t stream);
 // Fused outer pass of the 3D step (dc_outer3d.hip): the same work over 64 x 8
 // columns marching up chunks of `ch` planes.
 struct DcOuter3D {

@@ -4,20 +4,24 @@
 //
 // Per 64 x 8 tile (plus a one-cell ring, in LDS):
 //   cells   u = u_old + e (domain), the BC rule for first-layer ghosts
-//           (local rules only: no periodic faces), alpha(u), theta, dtau,
-//           d(alpha)/du; writes u and alpha of the tile and its ghost strips
+//           (local rules only: no periodic faces), alpha(u) in FP64; theta,
+//           the update coefficient C and d(alpha)/du in FP32; writes u of the
+//           tile and its ghost strips
 //   faces   q' = -abar (u1 - u0) on expand_lower(domain, axis) (FP64), and for
-//           the tile's faces the sweep coefficients P, M, W (k_lin_coefs)
+//           the tile's faces the sweep coefficients P, M, W (FP32)
 //   update  the residual of the reference discrete equations
 //           (residuals.jl:24-47) per tile cell, in FP64, stored in FP32 as the
 //           right-hand side of the correction equation; block partials of
 //           sum(r^2) and max|r| for the FP64 norm
-// Arithmetic per cell and per face is that of k_dc_update, k_dc_residual and
-// k_lin_coefs; each face is evaluated once instead of by every cell using it.
+// The FP64 arithmetic (u, alpha, abar, q', the residual and its norm) is that
+// of k_dc_update and k_dc_residual; each face is evaluated once instead of by
+// every cell using it.  alpha is not stored: the step's epilogue forms it
+// once from the final iterate.
 #include <limits>
 
 #include "cdiff_dev.h"
 #include "dc.h"
+#include "lin_coef_ops.h"
 
 namespace cdiff {
 
@@ -29,20 +33,18 @@ constexpr int OFXN = OTX + 1, OFYN = OTX + 2;
 constexpr int ONFX = OFXN * OCY, ONFY = OFYN * (OTY + 1);
 constexpr int ONT = OTX * OTY;
 
-__device__ __forceinline__ double gfac(const Ghost& gh, int f) {
+__device__ __forceinline__ float gfac(const Ghost& gh, int f) {
   const int m = gh.mode[f];
-  return m == G_COPY ? 1.0 : (m == G_NEG ? -1.0 : 0.0);
-}
-
-__device__ __forceinline__ double dadu(double t_exp, double a, double u) {
-  const double d = (u != 0.0 && isfinite(u)) ? t_exp * a / u : 0.0;
-  return isfinite(d) ? d : 0.0;
+  return m == G_COPY ? 1.0f : (m == G_NEG ? -1.0f : 0.0f);
 }
 
 }  // namespace
 
-__global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
-  __shared__ double sU[ONC], sA[ONC], sT[ONC], sDa[ONC];
+// UNIT: no density array and cp = 1 (alpha without its division by rho cp)
+template <bool UNIT>
+__global__ __launch_bounds__(ONT) __attribute__((amdgpu_waves_per_eu(8))) void k_dc_outer2d(DcOuter2D A) {
+  __shared__ double sU[ONC], sA[ONC];
+  __shared__ float sT[ONC], sDa[ONC];
   __shared__ double sX[ONFX], sY[ONFY];
   __shared__ double sRed[2][ONT / WAVE];
   const Geo& g = A.g;
@@ -60,6 +62,24 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
   const int x0 = g.lo[0] + bx * OTX, y0 = g.lo[1] + by * OTY;
   const int harm = A.harmonic;
   const double dt = A.ic.dt;
+  const IterConst32 k32 = iter_const32(A.ic, A.dtau_scale, A.theta_scale);
+  const float t_exp = (float)A.t_exp;
+
+  // inputs of the residual of this thread's cell, loaded before the ring
+  // work so that their latency overlaps it
+  const int tx = tid % OTX, tyy = tid / OTX;
+  const int i = x0 + tx, j = y0 + tyy;
+  const bool mine = i <= g.hi[0] && j <= g.hi[1];
+  const int64_t cm = mine ? at(g, i, j, 0) : 0;
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, m4 = 0.0, um = 0.0;
+  if (mine) {
+    m0 = A.c[0][cm];
+    m1 = A.c[1][cm];
+    m2 = A.c[2][cm];
+    m3 = A.c[3][cm];
+    m4 = A.c[4][cm];
+    um = A.uprev[cm];
+  }
 
   auto new_u = [&](int i, int r) {
     const int64_t c = at(g, i, r, 0);
@@ -73,7 +93,8 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
     if (t >= ONC) break;
     const int cx = t % OCX, cy = t / OCX;
     const int i = x0 - 1 + cx, r = y0 - 1 + cy;
-    double u = 0.0, a = 0.0, th = 0.0, da = 0.0;
+    double u = 0.0, a = 0.0;
+    float th = 0.0f, da = 0.0f;
     if (i >= g.lo[0] - 1 && i <= g.hi[0] + 1 && r >= g.lo[1] - 1 && r <= g.hi[1] + 1) {
       const int64_t c = at(g, i, r, 0);
       const bool gi = (i < g.lo[0]) | (i > g.hi[0]);
@@ -93,21 +114,17 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
         u = A.u_old[c];
       }
       const double rho = A.rho ? A.rho[c] : 1.0;
-      a = alpha_of<double>(A.kap, rho, u);
-      double dtau;
-      iter_params<double>(A.ic, rho, a, dtau, th);
-      dtau *= A.dtau_scale;
-      th *= A.theta_scale;
-      da = dadu(A.t_exp, a, u);
+      a = alpha_of<double, UNIT>(A.kap, rho, u);
+      float C;
+      iter_params32(k32, (float)rho, (float)a, C, th);
+      da = dadu32(t_exp, (float)a, (float)u);
       const bool own_r = cy >= 1 && cy <= OTY && r <= g.hi[1];
       const bool own_i = cx >= 1 && cx <= OTX && i <= g.hi[0];
       if (!gi && !gj && own_r && own_i) {
         A.u_new[c] = u;
-        A.a64[c] = a;
-        A.cc[c] = (float)(dtau / (1.0 + dtau / dt));
+        A.cc[c] = C;
       } else if ((gi && !gj && own_r) || (gj && !gi && own_i)) {
         A.u_new[c] = u;
-        A.a64[c] = a;
       }
     }
     sU[t] = u;
@@ -118,31 +135,26 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
   __syncthreads();
 
   // ---- faces: q' for the residual; P, M, W of the tile's faces
-  auto coefs = [&](int c0, int c1, double a0, double a1, double abar, bool lo_face, bool hi_face, int flo, int fhi,
-                   float* const* F, int64_t gc) {
-    const double tbar = face_mean<double>(harm, sT[c0], sT[c1]);
-    double P = abar, M = abar;
+  auto coefs = [&](int c0, int c1, double abar, bool lo_face, bool hi_face, int flo, int fhi, float* const* F,
+                   int64_t gc) {
+    float P = (float)abar, M = P;
     if (A.newton) {
-      const double s = a0 + a1;
-      double d0 = 0.5, d1 = 0.5;
-      if (harm) {
-        d0 = (s != 0.0) ? 2.0 * a1 * a1 / (s * s) : 0.0;
-        d1 = (s != 0.0) ? 2.0 * a0 * a0 / (s * s) : 0.0;
-      }
-      const double du = sU[c1] - sU[c0];
-      P -= d0 * sDa[c0] * du;
-      M += d1 * sDa[c1] * du;
+      float d0, d1;
+      mean_weights32(harm, (float)sA[c0], (float)sA[c1], d0, d1);
+      const float du = (float)(sU[c1] - sU[c0]);
+      P = __builtin_fmaf(-d0 * sDa[c0], du, P);
+      M = __builtin_fmaf(d1 * sDa[c1], du, M);
     }
     if (lo_face) {
-      M -= gfac(A.gh, flo) * P;
-      P = 0.0;
+      M = __builtin_fmaf(-gfac(A.gh, flo), P, M);
+      P = 0.0f;
     } else if (hi_face) {
-      P -= gfac(A.gh, fhi) * M;
-      M = 0.0;
+      P = __builtin_fmaf(-gfac(A.gh, fhi), M, P);
+      M = 0.0f;
     }
-    F[0][gc] = (float)P;
-    F[1][gc] = (float)M;
-    F[2][gc] = (float)(tbar / (1.0 + tbar));
+    F[0][gc] = P;
+    F[1][gc] = M;
+    F[2][gc] = damping32(harm, sT[c0], sT[c1]);
   };
 #pragma unroll
   for (int p = 0; p < (ONFX + ONFY + ONT - 1) / ONT; ++p) {
@@ -156,7 +168,7 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
         const double a0 = sA[c0], a1 = sA[c1], abar = face_mean<double>(harm, a0, a1);
         q = -abar * (sU[c1] - sU[c0]);
         if (cy >= 1 && cy <= OTY && (fx >= 1 || i == g.lo[0] - 1))
-          coefs(c0, c1, a0, a1, abar, i == g.lo[0] - 1, i == g.hi[0], 0, 1, A.fx, at(g, i, r, 0));
+          coefs(c0, c1, abar, i == g.lo[0] - 1, i == g.hi[0], 0, 1, A.fx, at(g, i, r, 0));
       }
       sX[t] = q;
     } else if (t < ONFX + ONFY) {
@@ -169,7 +181,7 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
         const double a0 = sA[c0], a1 = sA[c1], abar = face_mean<double>(harm, a0, a1);
         q = -abar * (sU[c1] - sU[c0]);
         if (fx >= 1 && fx <= OTX && (fy >= 1 || r == g.lo[1] - 1))
-          coefs(c0, c1, a0, a1, abar, r == g.lo[1] - 1, r == g.hi[1], 2, 3, A.fy, at(g, i, r, 0));
+          coefs(c0, c1, abar, r == g.lo[1] - 1, r == g.hi[1], 2, 3, A.fy, at(g, i, r, 0));
       }
       sY[u] = q;
     }
@@ -177,33 +189,30 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
   __syncthreads();
 
   // ---- residual of the tile's cells
-  const int tx = tid % OTX, tyy = tid / OTX;
-  const int i = x0 + tx, j = y0 + tyy;
   double sq = 0.0, mx = 0.0;
-  if (i <= g.hi[0] && j <= g.hi[1]) {
-    const int64_t c = at(g, i, j, 0);
+  if (mine) {
     const int xo = (tyy + 1) * OFXN + (tx + 1);
     const int yo = (tyy + 1) * OFYN + (tx + 1);
     const double xc = sX[xo], xw = sX[xo - 1], xn = sX[xo + OFXN], xnw = sX[xo + OFXN - 1];
     const double xs = sX[xo - OFXN], xsw = sX[xo - OFXN - 1];
     const double yc = sY[yo], ys = sY[yo - OFYN], ye = sY[yo + 1], yes = sY[yo - OFYN + 1];
     const double yw = sY[yo - 1], yws = sY[yo - OFYN - 1];
-    const double t1 = A.c[0][c] * (xc - xw);
-    const double t2 = A.c[1][c] * (yc - ys);
-    const double t3 = A.c[2][c] * ((xn + xnw) - (xs + xsw));
-    const double t4 = A.c[2][c] * ((ye + yes) - (yw + yws));
-    const double t5 = A.c[3][c] * (0.5 * (xc + xw));
-    const double t6 = A.c[4][c] * (0.5 * (yc + ys));
+    const double t1 = m0 * (xc - xw);
+    const double t2 = m1 * (yc - ys);
+    const double t3 = m2 * ((xn + xnw) - (xs + xsw));
+    const double t4 = m2 * ((ye + yes) - (yw + yws));
+    const double t5 = m3 * (0.5 * (xc + xw));
+    const double t6 = m4 * (0.5 * (yc + ys));
     const double D = t1 + t2 + t3 + t4 + t5 + t6;
-    const double un = sU[(tyy + 1) * OCX + tx + 1], um = A.uprev[c];
+    const double un = sU[(tyy + 1) * OCX + tx + 1];
     double du = un - um;
     bool approx = un == um;
     if (!approx && isfinite(un) && isfinite(um))
       approx = fabs(un - um) <= std::numeric_limits<double>::epsilon() * fmax(fabs(un), fabs(um));
     if (approx) du = copysign(0.0, du);
-    double r = -du / dt - D + (A.S ? A.S[c] : 0.0);
-    if (A.tzA) r = r + (A.tzA[c] + A.tzB[c] * sA[(tyy + 1) * OCX + tx + 1]);
-    A.r32[c] = (float)r;
+    double r = -du / dt - D + (A.S ? A.S[cm] : 0.0);
+    if (A.tzA) r = r + (A.tzA[cm] + A.tzB[cm] * sA[(tyy + 1) * OCX + tx + 1]);
+    A.r32[cm] = (float)r;
     sq = r * r;
     mx = fabs(r);
     if (r != r) mx = r;
@@ -224,6 +233,13 @@ __global__ __launch_bounds__(ONT) void k_dc_outer2d(DcOuter2D A) {
     A.psum[b] = s;
     A.pmax[b] = m;
   }
+}
+
+void launch_dc_outer2d(const DcOuter2D& A, int nblk, bool unit, hipStream_t stream) {
+  if (unit)
+    hipLaunchKernelGGL(k_dc_outer2d<true>, dim3(nblk), dim3(ONT), 0, stream, A);
+  else
+    hipLaunchKernelGGL(k_dc_outer2d<false>, dim3(nblk), dim3(ONT), 0, stream, A);
 }
 
 }  // namespace cdiff

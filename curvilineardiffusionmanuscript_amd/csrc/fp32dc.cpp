@@ -428,8 +428,10 @@ bool cdiff_solver::lin_ok() const {
 int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, double* u_new, const double* uprev,
                                 int iter, const cdiff_opts& o, bool finalize) {
   const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  if (!e) {
-    // alpha over the full array once (corners and deeper halo layers keep it)
+  if (!e && ndim == 3) {
+    // alpha over the full array once (corners and deeper halo layers keep it;
+    // the 3D pass refreshes the domain and the first ghost layer); the 2D pass
+    // keeps alpha in LDS only
     hipLaunchKernelGGL(k_alpha_full, grid_n(nfull), dim3(256), 0, stream, g, kap, (const double*)rho,
                        (const double*)nullptr, u_old, a64);
     LAUNCH_CHECK("k_alpha_full");
@@ -496,7 +498,6 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
   A.u_old = u_old;
   A.e = e;
   A.u_new = u_new;
-  A.a64 = a64;
   A.uprev = uprev;
   for (int c = 0; c < 5; ++c) A.c[c] = coef[c];
   A.S = S;
@@ -512,7 +513,7 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
   A.pmax = dc_pmax;
   const int nblk = (A.ntiles + 7) / 8 * 8;
   if (nblk > dc_nparts) return fail(CDIFF_HIP_ERROR, "partial-sum buffer too small");
-  hipLaunchKernelGGL(k_dc_outer2d, dim3(nblk), dim3(64 * DC_OUTER2D_TY), 0, stream, A);
+  launch_dc_outer2d(A, nblk, rho == nullptr && kap.cp == 1.0, stream);
   LAUNCH_CHECK("k_dc_outer2d");
   if (finalize) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl, (const double*)dc_psum,
@@ -682,10 +683,15 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     bool pending = false;
     int ebuf = 0;  // source buffer of the next inner launch
     std::vector<std::pair<int, int>> ends;  // (last iteration, buffer written)
-    const int seg_len = std::max(ichk, env_int("CDIFF_DC_SEG", 4 * ichk));
+    // the first segment runs to the check at or after the previous
+    // correction's sweep count, later ones one check each; one segment is
+    // always queued ahead of the host's read, so a correction that converges
+    // leaves at most one check's launches (which return at once) behind it
+    const int first_len = (int)std::min<int64_t>(
+        INT_MAX / 2, std::max<int64_t>(ichk, (std::max<int64_t>(dc_inner_guess, 1) + ichk - 1) / ichk * ichk));
     while (true) {
       const int s = it + 1;
-      const int e = std::min(cap, s + seg_len - 1);
+      const int e = (int)std::min<int64_t>(cap, (int64_t)s + (seg == 0 ? first_len : ichk) - 1);
       for (int t = s; t <= e;) {
         auto is_chk = [&](int x) { return fixed_inner > 0 ? (x == cap) : (x % ichk == 0 || x == cap); };
         int tl = t;
@@ -727,6 +733,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     HIP_TRY(hipMemcpyAsync(&ctrl_h[3], ctrl_in, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     const int ni = ctrl_h[3].final_iter > 0 ? ctrl_h[3].final_iter : it;
+    dc_inner_guess = ni;
     inner_total += ni;
     inner_checks += ctrl_h[3].nchecks;
     if (ctrl_h[3].status != 0 && (dc_cfl_scale > 1.0 || dc_dtau_scale != 1.0 || dc_theta_scale != 1.0)) {

@@ -47,6 +47,16 @@ __global__ void k_next_dt_partial(Geo g, const double* unew, const double* uold,
 __global__ void k_reduce_max(const double* part, int n, double* out);
 __global__ void k_next_dt_final(const double* mx, double dt, double muc, double growth, double* out);
 __global__ void k_plain_flux(Geo g, int axis, int harmonic, const double* u, const double* alpha, double* qp);
+// fused step-boundary passes (step_kernels.hip): STEP_THREADS threads per
+// block, at most STEP_BLOCKS blocks (one partial maximum per block)
+constexpr int STEP_THREADS = 256;
+constexpr int STEP_BLOCKS = 2048;
+__global__ void k_step_begin(Geo g, const double* T, double* u0, double* u1, double* up, const double* alpha,
+                             int positivity, Ctrl* ctrl, double* part);
+__global__ void k_step_end(Geo g, const double* u, double* T, int positivity, int cutoff, const double* up,
+                           const double* umax, double tol, double* part);
+void launch_dc_end(const Geo& g, const KappaLaw& kap, const double* rho, int harmonic, const double* u,
+                   double* ucopy, double* alpha, double* const q[3], hipStream_t stream);
 __global__ void k_residual_field(Geo g, double dt, const double* u, const double* uprev, const double* q0,
                                  const double* q1, const double* q2, Coefs cf, const double* S, const double* tzA,
                                  const double* tzB, const double* alpha, double* res);

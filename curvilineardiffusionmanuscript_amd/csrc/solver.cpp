@@ -427,11 +427,21 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
     warm_r0_ready = lin && !dirichlet_outer;
   }
 
-  // prologue: u = u_prev = T (PseudoTransientSolver.jl:229-230), validity (:238-250)
-  for (double* d : {u[0], u[1], u_prev}) {
-    hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)Tdev, d);
-    LAUNCH_CHECK("k_copy");
-  }
+  // prologue: u = u_prev = T (PseudoTransientSolver.jl:229-230), validity
+  // (:238-250) and maximum(u_prev) for next_dt, in one pass over T and alpha
+  const int nrows = g.n[1] * g.n[2];
+  const int step_blocks = std::min(nrows, STEP_BLOCKS);
+  auto begin_step = [&]() {
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, o.rtol, o.atol);
+    hipLaunchKernelGGL(k_step_begin, dim3(step_blocks), dim3(STEP_THREADS), 0, stream, g, (const double*)Tdev, u[0],
+                       u[1], u_prev, (const double*)alpha, o.enforce_positivity, ctrl,
+                       o.calculate_next_dt ? red + 8 : nullptr);
+    if (o.calculate_next_dt)
+      hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, stream, (const double*)(red + 8), step_blocks, red + 0);
+    if (S) hipLaunchKernelGGL(k_validate, grid_full(g), dim3(256), 0, stream, g, (const double*)S, 0, 16, ctrl);
+    hipLaunchKernelGGL(k_ctrl_validated, dim3(1), dim3(1), 0, stream, ctrl);
+  };
+  // the control block and validity again, the buffers kept (warm start)
   auto reset_and_validate = [&]() {
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, o.rtol, o.atol);
     hipLaunchKernelGGL(k_validate, grid_full(g), dim3(256), 0, stream, g, (const double*)alpha, o.enforce_positivity,
@@ -439,7 +449,7 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
     if (S) hipLaunchKernelGGL(k_validate, grid_full(g), dim3(256), 0, stream, g, (const double*)S, 0, 16, ctrl);
     hipLaunchKernelGGL(k_ctrl_validated, dim3(1), dim3(1), 0, stream, ctrl);
   };
-  reset_and_validate();
+  begin_step();
   LAUNCH_CHECK("prologue");
   if (dirichlet_outer)
     for (int b = 0; b < 2; ++b) {
@@ -458,12 +468,7 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
       if (std::getenv("CDIFF_DC_DEBUG"))
         std::fprintf(stderr, "[dc] defect correction gave up after %d rounds; FP64 APT from T^n\n", st.outer);
       fallback = true;
-      for (double* d : {u[0], u[1], u_prev}) {
-        hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)Tdev,
-                           d);
-        LAUNCH_CHECK("k_copy");
-      }
-      reset_and_validate();
+      begin_step();
       if (dirichlet_outer)
         for (int b = 0; b < 2; ++b) hipLaunchKernelGGL(k_halo_fill, grid_full(g), dim3(256), 0, stream, g, gh, u[b], 1);
       LAUNCH_CHECK("fallback prologue");
@@ -562,17 +567,13 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   last_iter = fi;
   if (dc) {
     // final iterate (with the BC ghosts of its own domain values) in u[dst];
-    // alpha of that state; the FP64 damped flux restarts from the plain flux
-    // of that state (the APT fixed point of q), so a following FP64 step
-    // warm-starts consistently
+    // alpha of that state over the whole array; the FP64 damped flux restarts
+    // from the plain flux of that state (the APT fixed point of q), so a
+    // following FP64 step warm-starts consistently
     dst = dc_final_buf;
     src = 1 - dst;
-    hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)u[dst],
-                       u[src]);
-    hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)a64, alpha);
-    for (int ax = 0; ax < ndim; ++ax)
-      hipLaunchKernelGGL(k_plain_flux, grid_full(g), dim3(256), 0, stream, g, ax, (int)(mean == CDIFF_MEAN_HARMONIC),
-                         (const double*)u[dst], (const double*)a64, q[ax][dst]);
+    double* const qd[3] = {q[0][dst], q[1][dst], ndim == 3 ? q[2][dst] : nullptr};
+    launch_dc_end(g, kap, rho, (int)(mean == CDIFF_MEAN_HARMONIC), u[dst], u[src], alpha, qd, stream);
     LAUNCH_CHECK("dc epilogue");
   } else if (o.subcycle_conductivity) {
     hipLaunchKernelGGL(k_alpha_full, grid_full(g), dim3(256), 0, stream, g, kap, (const double*)rho, pre,
@@ -581,12 +582,16 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
     hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)frozen[0],
                        alpha);
   }
-  hipLaunchKernelGGL(k_epilogue_u, grid_full(g), dim3(256), 0, stream, g, (const double*)u[dst], Tdev,
-                     o.enforce_positivity, o.apply_cutoff);
+  // T = u (positivity, cutoff!) and the relative change for next_dt in one pass
+  hipLaunchKernelGGL(k_step_end, dim3(step_blocks), dim3(STEP_THREADS), 0, stream, g, (const double*)u[dst], Tdev,
+                     o.enforce_positivity, o.apply_cutoff, (const double*)u_prev, (const double*)red, o.tol,
+                     o.calculate_next_dt ? red + 8 : nullptr);
   LAUNCH_CHECK("epilogue");
   if (o.calculate_next_dt) {
-    int rc = enqueue_next_dt(Tdev, u_prev, dt, o.tol, o.maximum_u_change, o.timestep_growth_factor);
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, stream, (const double*)(red + 8), step_blocks, red + 1);
+    hipLaunchKernelGGL(k_next_dt_final, dim3(1), dim3(1), 0, stream, (const double*)(red + 1), dt,
+                       o.maximum_u_change, o.timestep_growth_factor, red + 2);
+    LAUNCH_CHECK("next_dt");
     HIP_TRY(hipMemcpyAsync(red_h, red + 2, sizeof(double), hipMemcpyDeviceToHost, stream));
   }
   HIP_TRY(hipStreamSynchronize(stream));

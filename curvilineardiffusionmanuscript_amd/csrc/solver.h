@@ -53,6 +53,7 @@ struct cdiff_solver {
   float* da32 = nullptr;    // d(alpha)/du at the FP64 iterate (Newton correction)
   bool dc_newton = false;
   double* a64 = nullptr;    // alpha of the FP64 iterate
+  int64_t dc_inner_guess = 48;  // sweeps of the last correction solve (length of the first segment)
   cdiff::Ghost gh32{};      // ghost rules of the correction
   cdiff::Ctrl* ctrl_in = nullptr;
   double* dc_psum = nullptr;

@@ -45,6 +45,10 @@ int cdiff_solver::learn_features(double dt, void* X0, double* ts_dev) {
   dc_Vpdtau = o.cfl * smin;
   // alpha(T^n) and the FP64 residual of T^n as the start of a step (u = u_prev = T^n)
   if ((rc = dc_outer_pass(dt, Tdev, nullptr, u[1], Tdev, 1, o, false))) return rc;
+  hipLaunchKernelGGL(k_alpha_full, dim3((unsigned)(((int64_t)g.n[0] * g.n[1] * g.n[2] + 255) / 256)), dim3(256), 0,
+                     stream, g, kap,
+                     (const double*)rho, (const double*)nullptr, (const double*)Tdev, a64);
+  LAUNCH_CHECK("k_alpha_full");
   launch_domain_maxabs(g, Tdev, learn_scratch, ts_dev, stream);
   launch_domain_maxabs32(g, base32, dt, learn_scratch, ts_dev + 1, stream);
   LAUNCH_CHECK("feature scales");
