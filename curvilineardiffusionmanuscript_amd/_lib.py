@@ -24,6 +24,9 @@ FIELD_Q, FIELD_QPRIME, FIELD_CACHE, FIELD_DENSITY = 8, 12, 16, 20
 
 EXPORTS = (
     "cdiff_default_opts",
+    "cdiff_default_tuning",
+    "cdiff_set_tuning",
+    "cdiff_get_tuning",
     "cdiff_create",
     "cdiff_set_metrics",
     "cdiff_set_nodes",
@@ -93,6 +96,17 @@ class Opts(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """cdiff_tuning: kernel variants and defect-correction parameters."""
+
+    _fields_ = [(n, C.c_int32) for n in (
+        "sweep2d", "tile_rows", "march_rows64", "pairs", "sweep3d", "planes3d", "seg_min", "seg_max",
+        "dc_coefficient_form", "dc_newton", "dc_fused3d", "dc_depth2d", "dc_layout2d", "dc_march_rows",
+        "dc_launch_iters", "dc_depth3d", "dc_planes3d", "dc_planes3d_pair", "dc_outer_planes3d", "dc_check",
+        "dc_max_outer", "verbose", "exact_pow")] + [(n, C.c_double) for n in (
+        "dc_eta", "dc_floor", "dc_cfl_scale", "dc_dtau_scale", "dc_theta_scale")]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("rel_err", C.c_double),
@@ -127,6 +141,10 @@ def load(path: str = LIB_PATH):
     L = C.CDLL(path)
     L.cdiff_default_opts.argtypes = [C.POINTER(Opts)]
     L.cdiff_default_opts.restype = None
+    L.cdiff_default_tuning.argtypes = [C.POINTER(Tuning)]
+    L.cdiff_default_tuning.restype = None
+    L.cdiff_set_tuning.argtypes = [_vp, C.POINTER(Tuning)]
+    L.cdiff_get_tuning.argtypes = [_vp, C.POINTER(Tuning)]
     L.cdiff_create.argtypes = [C.c_int, C.POINTER(C.c_int64), C.c_int, C.c_int, C.POINTER(BC), C.POINTER(Kappa), C.c_double, C.POINTER(_vp)]
     L.cdiff_set_metrics.argtypes = [_vp, C.POINTER(_dp), C.POINTER(_dp), C.c_double, _dp]
     L.cdiff_set_nodes.argtypes = [_vp, C.POINTER(_dp), C.c_double, _dp]
@@ -172,7 +190,8 @@ def load(path: str = LIB_PATH):
     L.cdiff_set_warm_start.argtypes = [_vp, _vp]
     L.cdiff_net_set_gate.argtypes = [_vp, C.c_float]
     for name in EXPORTS:
-        if name not in ("cdiff_default_opts", "cdiff_last_error", "cdiff_status_string", "cdiff_net_last_error"):
+        if name not in ("cdiff_default_opts", "cdiff_default_tuning", "cdiff_last_error", "cdiff_status_string",
+                        "cdiff_net_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

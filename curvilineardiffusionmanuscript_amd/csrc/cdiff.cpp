@@ -142,11 +142,10 @@ static void set_law(cdiff_solver* h, const cdiff_kappa* kappa, double cp) {
   k.t_int = int_exp(kappa->t_exp);
   k.rho_int = int_exp(kappa->rho_exp);
   // T^(n+1/2) by x^n*sqrt(x) unless the exact pow() is requested
-  const char* ex = std::getenv("CDIFF_EXACT_POW");
   const double f2 = 2.0 * kappa->t_exp;
   const bool half = f2 == std::floor(f2) && kappa->t_exp != std::floor(kappa->t_exp) && kappa->t_exp > 0 &&
                     kappa->t_exp < 4;
-  k.t_half = (half && !(ex && *ex == '1')) ? (int)std::floor(kappa->t_exp) : -1;
+  k.t_half = (half && !h->tune.exact_pow) ? (int)std::floor(kappa->t_exp) : -1;
   h->cp = cp;
 }
 
@@ -173,6 +172,79 @@ void cdiff_default_opts(cdiff_opts* o) {
   o->timestep_growth_factor = 1.1;
   o->precision = CDIFF_PREC_FP64;
   o->inner_iters = 0;
+}
+
+void cdiff_default_tuning(cdiff_tuning* t) {
+  if (!t) return;
+  std::memset(t, 0, sizeof *t);
+  t->sweep2d = 0;
+  t->tile_rows = 8;
+  t->march_rows64 = 0;
+  t->pairs = 1;
+  t->sweep3d = 0;
+  t->planes3d = 0;
+  t->seg_min = 4;
+  t->seg_max = 256;
+  t->dc_coefficient_form = 1;
+  t->dc_newton = 1;
+  t->dc_fused3d = 1;
+  t->dc_depth2d = 6;
+  t->dc_layout2d = 2;
+  t->dc_march_rows = 0;
+  t->dc_launch_iters = 0;
+  t->dc_depth3d = 2;
+  t->dc_planes3d = 0;
+  t->dc_planes3d_pair = 0;
+  t->dc_outer_planes3d = 32;
+  t->dc_check = 0;
+  t->dc_max_outer = 100;
+  t->verbose = 0;
+  t->exact_pow = 0;
+  t->dc_eta = 0.0;
+  t->dc_floor = 0.3;
+  t->dc_cfl_scale = 1.2;
+  t->dc_dtau_scale = 1.3;
+  t->dc_theta_scale = 1.0;
+}
+
+int cdiff_set_tuning(cdiff_solver* h, const cdiff_tuning* t) {
+  if (!h || !t) return CDIFF_BAD_ARG;
+  const cdiff_tuning& v = *t;
+  auto bad = [&](const char* what) { return h->fail(CDIFF_BAD_ARG, std::string("cdiff_set_tuning: ") + what); };
+  if (v.sweep2d < 0 || v.sweep2d > 1) return bad("sweep2d must be 0 or 1");
+  if (v.tile_rows != 4 && v.tile_rows != 8 && v.tile_rows != 16) return bad("tile_rows must be 4, 8 or 16");
+  if (v.sweep3d < 0 || v.sweep3d > 1) return bad("sweep3d must be 0 or 1");
+  if (v.march_rows64 < 0 || v.planes3d < 0 || v.dc_march_rows < 0 || v.dc_planes3d < 0 ||
+      v.dc_planes3d_pair < 0 || v.dc_outer_planes3d < 1)
+    return bad("chunk sizes must be >= 0 (>= 1 for dc_outer_planes3d)");
+  if (v.seg_min < 1 || v.seg_max < v.seg_min) return bad("need 1 <= seg_min <= seg_max");
+  if (v.dc_depth2d != 2 && v.dc_depth2d != 4 && v.dc_depth2d != 6 && v.dc_depth2d != 8)
+    return bad("dc_depth2d must be 2, 4, 6 or 8");
+  if (v.dc_layout2d < 0 || v.dc_layout2d > 2) return bad("dc_layout2d must be 0, 1 or 2");
+  if (v.dc_depth3d < 1 || v.dc_depth3d > 2) return bad("dc_depth3d must be 1 or 2");
+  if (v.dc_launch_iters < 0 || v.dc_check < 0 || v.dc_max_outer < 1) return bad("dc counts out of range");
+  if (!(v.dc_eta >= 0.0 && v.dc_eta < 1.0) || !(v.dc_floor >= 0.0) || !(v.dc_cfl_scale > 0.0) ||
+      !(v.dc_dtau_scale > 0.0) || !(v.dc_theta_scale > 0.0))
+    return bad("dc_eta in [0, 1), dc_floor >= 0 and positive scales required");
+  const bool pow_changed = (v.exact_pow != 0) != (h->tune.exact_pow != 0);
+  h->tune = v;
+  if (pow_changed) {
+    cdiff_kappa k{h->kap.kappa0, h->kap.t_exp, h->kap.rho_exp};
+    set_law(h, &k, h->cp);
+  }
+  // the FP64 launch geometry and the correction parameters are chosen again
+  h->configure_sweep();
+  int rc = h->ensure_parts();
+  if (rc) return rc;
+  h->dc_cfl_scale = -1.0;
+  h->dc_dtau_scale_dflt = h->dc_theta_scale_dflt = -1.0;
+  return CDIFF_OK;
+}
+
+int cdiff_get_tuning(const cdiff_solver* h, cdiff_tuning* t) {
+  if (!h || !t) return CDIFF_BAD_ARG;
+  *t = h->tune;
+  return CDIFF_OK;
 }
 
 const char* cdiff_status_string(int s) {
@@ -219,6 +291,7 @@ int cdiff_create(int ndim, const int64_t* dims_full, int nhalo, int mean, const 
     return h->fail(CDIFF_BAD_ARG, "mesh too large: a field would hold 2^31 or more elements");
   }
   h->N_full = (double)g.n[0] * g.n[1] * g.n[2];
+  cdiff_default_tuning(&h->tune);
   set_law(h, kappa, cp);
   int rc = build_ghost(h);
   if (rc) {
@@ -252,10 +325,7 @@ int cdiff_create(int ndim, const int64_t* dims_full, int nhalo, int mean, const 
   HIP_TRY(hipHostMalloc((void**)&h->red_h, sizeof(double) * 8, hipHostMallocDefault));
   for (int e = 0; e < 2; ++e) HIP_TRY(hipEventCreateWithFlags(&h->ev[e], hipEventDisableTiming));
   h->configure_sweep();
-  HIP_TRY(hipMalloc((void**)&h->part_sum, sizeof(double) * h->nparts));
-  h->owned.push_back(h->part_sum);
-  HIP_TRY(hipMalloc((void**)&h->part_max, sizeof(double) * h->nparts));
-  h->owned.push_back(h->part_max);
+  if ((rc = h->ensure_parts())) return rc;
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0.0, 0.0);
   LAUNCH_CHECK("k_ctrl_reset");
   HIP_TRY(hipStreamSynchronize(h->stream));

@@ -46,16 +46,6 @@ using namespace cdiff;
 
 namespace {
 
-int env_int(const char* name, int dflt) {
-  const char* v = std::getenv(name);
-  return (v && *v) ? std::atoi(v) : dflt;
-}
-
-double env_double(const char* name, double dflt) {
-  const char* v = std::getenv(name);
-  return (v && *v) ? std::atof(v) : dflt;
-}
-
 dim3 grid_n(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace
@@ -77,7 +67,7 @@ int cdiff_solver::ensure_fp32() {
       if ((rc = alloc32(&q32[a][b]))) return rc;
   if ((rc = alloc32(&base32))) return rc;
   if ((rc = alloc32(&a32))) return rc;
-  dc_newton = env_int("CDIFF_DC_NEWTON", 1) != 0 && kap.t_exp != 0.0 && (ndim == 2 || lin_ok());
+  dc_newton = tune.dc_newton != 0 && kap.t_exp != 0.0 && (ndim == 2 || lin_ok());
   if (dc_newton) {
     if ((rc = alloc32(&ub32))) return rc;
     if ((rc = alloc32(&da32))) return rc;
@@ -134,10 +124,9 @@ int cdiff_solver::ensure_fp32() {
     // VGPRs leave two wavefronts per SIMD).  On wave-aligned 52 x 20 tiles
     // the same launch takes 79 us per iteration (92 at depth 4, 83 at
     // depth 8; 100 at depth 4 on 64 x 16 tiles): tiles re-read their ring
-    lin_k = env_int("CDIFF_LIN_K", 6);
-    if (lin_k != 2 && lin_k != 4 && lin_k != 6 && lin_k != 8) lin_k = 6;
+    lin_k = tune.dc_depth2d;
     const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
-    lin_layout = std::max(0, std::min(2, env_int("CDIFF_LIN2_LAYOUT", 2)));
+    lin_layout = tune.dc_layout2d;
     if (lin_layout == 2 && lin_k == 2) lin_layout = 1;  // the marching sweep runs 4, 6 or 8 iterations
     int cols = 64, rows = 16;
     // (launches of fewer than lin_k iterations use the wave-aligned tiles)
@@ -152,23 +141,23 @@ int cdiff_solver::ensure_fp32() {
       int want = std::max(1, 12 * ncu / march_strips);
       if (want >= 8) want -= want % 8;
       march_rows = std::max(2 * lin_k, (nj + want - 1) / want);
-      march_rows = std::max(2 * lin_k, env_int("CDIFF_MARCH_ROWS", march_rows));
+      if (tune.dc_march_rows > 0) march_rows = std::max(2 * lin_k, (int)tune.dc_march_rows);
       march_chunks = (nj + march_rows - 1) / march_rows;
     }
   } else if (lin) {
     // 3D: one or two iterations per launch; 64 x 8 columns over chunks of planes
-    lin_k = std::max(1, std::min(2, env_int("CDIFF_LIN3_K", 2)));
+    lin_k = tune.dc_depth3d;
     const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1, nk = g.hi[2] - g.lo[2] + 1;
     // about two 512-thread blocks per CU (256^3: 64 planes per chunk)
     const int64_t cols = (int64_t)((ni + 63) / 64) * ((nj + 7) / 8);
     const int dflt = (int)std::min<int64_t>(256, std::max<int64_t>(4, (cols * nk + 2 * ncu - 1) / (2 * ncu)));
-    lin3_ch = std::max(1, std::min(nk, env_int("CDIFF_LIN3_CH", dflt)));
+    lin3_ch = std::max(1, std::min(nk, tune.dc_planes3d > 0 ? (int)tune.dc_planes3d : dflt));
     lin3_strips = (ni + 63) / 64;
     lin_chunks = (nk + lin3_ch - 1) / lin3_ch;
     lin_tiles = lin3_strips * ((nj + 7) / 8) * lin_chunks;
     // the two-iteration sweep runs one 832-thread block per CU (256^3: 128 planes per chunk)
     const int dflt2 = (int)std::min<int64_t>(256, std::max<int64_t>(4, (cols * nk + ncu - 1) / ncu));
-    lin3b_ch = std::max(1, std::min(nk, env_int("CDIFF_LIN3B_CH", dflt2)));
+    lin3b_ch = std::max(1, std::min(nk, tune.dc_planes3d_pair > 0 ? (int)tune.dc_planes3d_pair : dflt2));
     lin3b_chunks = (nk + lin3b_ch - 1) / lin3b_ch;
     lin3b_tiles = lin3_strips * ((nj + 7) / 8) * lin3b_chunks;
     lin3b_nblocks = (lin3b_tiles + 7) / 8 * 8;
@@ -419,7 +408,7 @@ int cdiff_solver::launch_lin(double dt, int src, bool check, int nit, bool zero_
 }
 
 bool cdiff_solver::lin_ok() const {
-  if (env_int("CDIFF_DC_LIN", 1) == 0) return false;
+  if (!tune.dc_coefficient_form) return false;
   for (int f = 0; f < 2 * ndim; ++f)
     if (bcs[f].kind == CDIFF_BC_PERIODIC) return false;
   return true;
@@ -448,7 +437,7 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
     B.theta_scale = dc_theta_scale;
     B.harmonic = mean == CDIFF_MEAN_HARMONIC;
     B.newton = dc_newton;
-    B.ch = std::max(1, std::min(nk, env_int("CDIFF_OUT3_CH", 32)));
+    B.ch = std::max(1, std::min(nk, (int)tune.dc_outer_planes3d));
     B.stripsI = (ni + 63) / 64;
     B.chunks = (nk + B.ch - 1) / B.ch;
     B.ntiles = B.stripsI * ((nj + 7) / 8) * B.chunks;
@@ -539,7 +528,7 @@ int cdiff_solver::dc_finalize_outer(int iter, const cdiff_opts& o) {
 // residual of the iterate in u[dstbuf] after u[dstbuf] = u[srcbuf] + e
 int cdiff_solver::dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o) {
   const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  if (lin && (ndim == 2 || env_int("CDIFF_DC_FUSED3", 1) != 0))
+  if (lin && (ndim == 2 || tune.dc_fused3d))
     return dc_outer_pass(dt, u[srcbuf], e, u[dstbuf], u_prev, iter, o, true);
   hipLaunchKernelGGL(k_dc_update, grid_n(nfull), dim3(256), 0, stream, g, gh, kap, (const double*)rho,
                      (const double*)u[srcbuf], e, u[dstbuf], a64, a32, (dc_newton && !lin) ? ub32 : nullptr,
@@ -567,26 +556,26 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   // fewer sweeps in total (256^3 spots: 48 -> 32 sweeps per step); in 2D
   // every second six-iteration launch checks (a launch that ends early on a
   // check costs nearly a full one)
-  const double eta = env_double("CDIFF_DC_ETA", ndim == 3 ? 1e-2 : 1e-3);
+  const double eta = tune.dc_eta > 0.0 ? tune.dc_eta : (ndim == 3 ? 1e-2 : 1e-3);
   // the last correction only needs to reach the outer tolerance: inner
   // sweeps stop at floor_frac * atol at the latest
-  const double floor_frac = env_double("CDIFF_DC_FLOOR", 0.3);
-  const int ichk = std::max(1, env_int("CDIFF_DC_CHECK", ndim == 3 ? 2 : 2 * lin_k));
-  const int max_outer = std::max(1, env_int("CDIFF_DC_MAX_OUTER", 100));
+  const double floor_frac = tune.dc_floor;
+  const int ichk = tune.dc_check > 0 ? (int)tune.dc_check : (ndim == 3 ? 2 : 2 * lin_k);
+  const int max_outer = tune.dc_max_outer;
   const int fixed_inner = o.inner_iters;
   const int64_t total_cap = (int64_t)o.max_iter;
   const bool pairs = pairs_ok();
-  const int lin_nit = std::max(1, std::min(lin_k, env_int("CDIFF_LIN_NIT", lin_k)));
+  const int lin_nit = std::max(1, std::min(lin_k, tune.dc_launch_iters > 0 ? (int)tune.dc_launch_iters : lin_k));
   // pseudo-time step of the correction sweeps: the fixed point does not depend
   // on it, so it may differ from the FP64 APT's (scale 1 = the reference's)
-  if (dc_cfl_scale < 0) dc_cfl_scale = env_double("CDIFF_DC_CFL_SCALE", 1.2);
+  if (dc_cfl_scale < 0) dc_cfl_scale = tune.dc_cfl_scale;
   dc_Vpdtau = Vpdtau * dc_cfl_scale;
   // and on top of that a 1.3x larger dtau with theta unchanged: measured on
   // the 4096^2 and 1024^2 hot disks, 4096^2 and 2048^2 blobs and 256^3 spots,
   // 5-12 % fewer sweeps per step (a scale of 2 diverged on some of them)
   if (dc_dtau_scale_dflt < 0) {
-    dc_dtau_scale_dflt = env_double("CDIFF_DC_DTAU_SCALE", 1.3);
-    dc_theta_scale_dflt = env_double("CDIFF_DC_THETA_SCALE", 1.0);
+    dc_dtau_scale_dflt = tune.dc_dtau_scale;
+    dc_theta_scale_dflt = tune.dc_theta_scale;
   }
   dc_dtau_scale = dc_dtau_scale_dflt;
   dc_theta_scale = dc_theta_scale_dflt;
@@ -626,7 +615,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     HIP_TRY(hipStreamSynchronize(stream));
   }
   st->warm_start = 0;
-  const bool dc_debug = env_int("CDIFF_DC_DEBUG", 0) != 0;
+  const bool dc_debug = tune.verbose != 0;
   if (dc_debug) std::fprintf(stderr, "[dc] step dt=%.3e: residual of T^n %.3e\n", dt, ctrl_h[2].abs_);
   if (warm_pending && lin && !ctrl_h[2].done) {
     bool used = false;

@@ -79,33 +79,43 @@ std::string julia_float(double x) {
 
 inline bool is_check(int it, int m, int k) { return (it >= m && it % k == 0) || it == 1; }
 
-int env_int(const char* name, int dflt) {
-  const char* v = std::getenv(name);
-  if (!v || !*v) return dflt;
-  return std::atoi(v);
-}
-
 }  // namespace
 
+int cdiff_solver::ensure_parts() {
+  if (nparts <= part_cap) return CDIFF_OK;
+  double* ps = nullptr;
+  double* pm = nullptr;
+  HIP_TRY(hipMalloc((void**)&ps, sizeof(double) * nparts));
+  HIP_TRY(hipMalloc((void**)&pm, sizeof(double) * nparts));
+  for (double* old : {part_sum, part_max})
+    if (old) {
+      HIP_TRY(hipStreamSynchronize(stream));
+      owned.erase(std::remove(owned.begin(), owned.end(), (void*)old), owned.end());
+      HIP_TRY(hipFree(old));
+    }
+  part_sum = ps;
+  part_max = pm;
+  owned.push_back(ps);
+  owned.push_back(pm);
+  part_cap = nparts;
+  return CDIFF_OK;
+}
+
 void cdiff_solver::configure_sweep() {
-  {
-    const char* v = std::getenv("CDIFF_SWEEP2D");
-    tile2d = !(v && std::string(v) == "march");
-  }
+  tile2d = tune.sweep2d == 0;
   const int ni = g.hi[0] - g.lo[0] + 1;
   const int nj = g.hi[1] - g.lo[1] + 1;
   if (ndim == 2 && tile2d) {
     // LDS-tiled kernel: 64 x 4 cells per block
     strips = (ni + 63) / 64;
-    int ty = env_int("CDIFF_TILE_TY", 8);
-    if (ty != 4 && ty != 8 && ty != 16) ty = 8;
+    const int ty = tune.tile_rows;
     ch = ty;
     chunks = (nj + ty - 1) / ty;
     ntiles = strips * chunks;
     nblocks = (ntiles + 7) / 8 * 8;
   } else if (ndim == 2) {
     strips = (ni + OWN - 1) / OWN;
-    int c = env_int("CDIFF_CH", 0);
+    int c = tune.march_rows64;
     if (c <= 0) {
       // aim for ~4096 waves (16 per CU), rows per chunk between 8 and 64
       const int64_t cells = (int64_t)strips * nj;
@@ -117,11 +127,8 @@ void cdiff_solver::configure_sweep() {
     nblocks = (ntiles + 7) / 8 * 8;
   } else {
     const int nk = g.hi[2] - g.lo[2] + 1;
-    {
-      const char* v = std::getenv("CDIFF_SWEEP3D");
-      col3d = !(v && std::string(v) == "lanes");
-    }
-    int c = env_int("CDIFF_CH3", 0);
+    col3d = tune.sweep3d == 0;
+    int c = tune.planes3d;
     if (col3d) {
       strips3[0] = (ni + 63) / 64;
       strips3[1] = (nj + 7) / 8;
@@ -151,7 +158,7 @@ void cdiff_solver::configure_sweep() {
 }
 
 bool cdiff_solver::pairs_ok() const {
-  if (ndim != 2 || !tile2d || ch != 8 || env_int("CDIFF_PAIRS", 1) == 0) return false;
+  if (ndim != 2 || !tile2d || ch != 8 || !tune.pairs) return false;
   for (int f = 0; f < 4; ++f)
     if (bcs[f].kind == CDIFF_BC_PERIODIC) return false;
   return true;
@@ -268,8 +275,8 @@ int cdiff_solver::enqueue_next_dt(const double* unew, const double* uold, double
 int cdiff_solver::run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int /*first_iter_base*/) {
   int m, k;
   schedule(last_iter, m, k);
-  const int lmin = std::max(1, env_int("CDIFF_SEG_MIN", 4));
-  const int lmax = std::max(lmin, env_int("CDIFF_SEG_MAX", 256));
+  const int lmin = tune.seg_min;
+  const int lmax = tune.seg_max;
   const int64_t limit = (int64_t)o.max_iter + 1;
 
   const bool pairs = pairs_ok();
@@ -465,7 +472,7 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
       // the defect correction gave up: redo the step with the FP64 APT from
       // T^n (Tdev, u_prev and the FP64 fluxes are untouched by the correction
       // loop), so the step ends like an FP64 step would
-      if (std::getenv("CDIFF_DC_DEBUG"))
+      if (tune.verbose)
         std::fprintf(stderr, "[dc] defect correction gave up after %d rounds; FP64 APT from T^n\n", st.outer);
       fallback = true;
       begin_step();

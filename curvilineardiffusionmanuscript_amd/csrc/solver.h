@@ -21,6 +21,7 @@ struct cdiff_solver {
   double L = 0, spacing[3] = {0, 0, 0};
   bool metrics_set = false;
   double N_full = 0;
+  cdiff_tuning tune{};  // kernel variants and defect-correction parameters (cdiff_set_tuning)
 
   // FP64 state
   double* u[2] = {nullptr, nullptr};
@@ -35,6 +36,7 @@ struct cdiff_solver {
   double* part_sum = nullptr;
   double* part_max = nullptr;
   int nparts = 0;
+  int part_cap = 0;  // capacity of part_sum / part_max
   double* red = nullptr;
   double* red_h = nullptr;
   cdiff::Ctrl* ctrl = nullptr;
@@ -145,6 +147,7 @@ struct cdiff_solver {
   int warm_start_apply(double dt, const cdiff_opts& o, bool* used);
   int hip_fail(hipError_t e, const char* what);
   void configure_sweep();
+  int ensure_parts();  // part_sum / part_max hold nparts entries
   int step(double dt, const cdiff_opts& o, cdiff_stats* stats, double* next_dt);
   int run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int first_iter_base);
   int enqueue_next_dt(const double* unew, const double* uold, double dt, double tol, double muc, double growth);

@@ -90,7 +90,7 @@ int cdiff_solver::warm_start_apply(double dt, const cdiff_opts& o, bool* used) {
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
-  if (std::getenv("CDIFF_DC_DEBUG"))
+  if (tune.verbose)
     std::fprintf(stderr, "[dc] residual of the predicted start %.3e (of T^n %.3e)\n", ctrl_h[2].abs_, r0);
   // the back-off: after a prediction that did not at least halve the
   // residual, the network sits out 3, 7, 15 (at most 16) steps.  A

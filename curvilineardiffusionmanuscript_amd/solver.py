@@ -217,12 +217,14 @@ class PseudoTransientSolver:
     """Device-resident APT solver state (PseudoTransientSolver.jl:28-46)."""
 
     def __init__(self, mesh, bcs, *, backend=None, mean="harmonic", T=np.float64, metrics=None, axisymmetric=False,
-                 **kwargs):
+                 tuning=None, **kwargs):
         """``metrics``: "device" builds the metric arrays on the GPU from the
         mesh's ghost-inclusive nodes (``mesh.nodes_full()``, the default when the
         mesh has them), "host" uploads the mesh's own metric arrays.
         ``axisymmetric``: 2D RZ geometry with x = r, y = z (the operator gains
-        (1/r) dT/dr; r = the cell centroid's x); not in the reference APT."""
+        (1/r) dT/dr; r = the cell centroid's x); not in the reference APT.
+        ``tuning``: overrides of the cdiff_tuning fields (kernel variants and
+        defect-correction parameters, see ``set_tuning``)."""
         if T not in (np.float64, float, "Float64"):
             raise TypeError("the drop-in path keeps the reference's Float64 state (see precision= for FP32 sweeps)")
         lib = L.load()
@@ -279,11 +281,35 @@ class PseudoTransientSolver:
         else:
             raise ValueError('metrics must be "device" or "host"')
         self.metrics = metrics
+        if tuning:
+            self.set_tuning(**tuning)
         self._rho_id = None
         self.iterators = SimpleNamespace(
             domain=SimpleNamespace(cartesian=mesh.iterators.cell.domain),
             full=SimpleNamespace(cartesian=mesh.iterators.cell.full),
         )
+
+    # -- tuning ----------------------------------------------------------------
+    def get_tuning(self) -> dict:
+        """The solver's cdiff_tuning as a dict."""
+        t = L.Tuning()
+        _check(L.load().cdiff_get_tuning(self._h, C.byref(t)), self._h)
+        return {n: getattr(t, n) for n, _ in L.Tuning._fields_}
+
+    def set_tuning(self, **overrides):
+        """Change fields of the solver's cdiff_tuning (no counterpart in the
+        reference): kernel variants that give the same bits (sweep2d,
+        tile_rows, pairs, dc_layout2d, dc_depth2d, dc_launch_iters, ...) and the
+        defect-correction parameters (dc_eta, dc_cfl_scale, ...).  Unknown
+        names raise; values out of range raise SolverAssertionError."""
+        t = L.Tuning()
+        _check(L.load().cdiff_get_tuning(self._h, C.byref(t)), self._h)
+        names = {n for n, _ in L.Tuning._fields_}
+        for k, v in overrides.items():
+            if k not in names:
+                raise TypeError(f"unknown tuning field {k!r}")
+            setattr(t, k, v)
+        _check(L.load().cdiff_set_tuning(self._h, C.byref(t)), self._h)
 
     # -- lifetime --------------------------------------------------------------
     def close(self):

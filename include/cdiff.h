@@ -147,6 +147,51 @@ typedef enum {
 
 void cdiff_default_opts(cdiff_opts *opts);
 
+/* Kernel variants and the parameters of the FP32 defect correction.  No
+ * counterpart in the reference, whose only configuration surface is step!'s
+ * keywords (PseudoTransientSolver.jl:188-202): cdiff_default_tuning gives the
+ * values measured best on MI355X.  The FP64 variants give bit-identical
+ * results, and so do the correction sweep's layouts and depths (the tests
+ * compare them); the defect-correction parameters change only the path to
+ * the same FP64 tolerance.  0 selects the automatic choice where noted. */
+typedef struct {
+  /* FP64 APT */
+  int32_t sweep2d;          /* 0: LDS tiles (default), 1: row-marching strips */
+  int32_t tile_rows;        /* rows per 2D tile: 4, 8 (default) or 16 */
+  int32_t march_rows64;     /* rows per chunk of the row-marching strips, 0 = auto */
+  int32_t pairs;            /* 1: two iterations per 2D launch where possible (default), 0: one */
+  int32_t sweep3d;          /* 0: 64 x 8 columns through LDS (default), 1: one wavefront per row */
+  int32_t planes3d;         /* planes per 3D chunk, 0 = auto */
+  int32_t seg_min, seg_max; /* launches between host reads of the convergence flag: 4, 256 */
+  /* FP32 defect correction */
+  int32_t dc_coefficient_form; /* 1: coefficient-form sweeps where no face is periodic (default), 0: conductivity form */
+  int32_t dc_newton;        /* 1: d(alpha)/du terms in the correction operator (default) */
+  int32_t dc_fused3d;       /* 1: fused 3D outer pass (default), 0: separate kernels */
+  int32_t dc_depth2d;       /* iterations per 2D correction launch: 2, 4, 6 (default) or 8 */
+  int32_t dc_layout2d;      /* 0: 64-column tiles, 1: wave-aligned tiles, 2: marching strips (default) */
+  int32_t dc_march_rows;    /* rows per marching chunk, 0 = auto */
+  int32_t dc_launch_iters;  /* iterations per launch, 1 .. depth; 0 = the depth */
+  int32_t dc_depth3d;       /* iterations per 3D correction launch: 1 or 2 (default) */
+  int32_t dc_planes3d;      /* planes per chunk of the one-iteration 3D sweep, 0 = auto */
+  int32_t dc_planes3d_pair; /* planes per chunk of the two-iteration 3D sweep, 0 = auto */
+  int32_t dc_outer_planes3d; /* planes per chunk of the fused 3D outer pass: 32 */
+  int32_t dc_check;         /* correction sweeps between checks, 0 = auto (2 x depth in 2D, 2 in 3D) */
+  int32_t dc_max_outer;     /* correction rounds before the step is redone by the FP64 APT: 100 */
+  int32_t verbose;          /* 1: per-round diagnostics of the defect correction on stderr */
+  int32_t exact_pow;        /* 1: pow() for T^(n+1/2) instead of T^n sqrt(T) (within 2 ulp) */
+  double dc_eta;            /* inner solves stop at eta x the outer residual; 0 = auto (1e-3 2D, 1e-2 3D) */
+  double dc_floor;          /* ... or at floor x atol: 0.3 */
+  double dc_cfl_scale;      /* pseudo-time CFL of the correction sweeps over the step's: 1.2 */
+  double dc_dtau_scale;     /* factor on their dtau: 1.3 */
+  double dc_theta_scale;    /* factor on their theta: 1.0 */
+} cdiff_tuning;
+
+void cdiff_default_tuning(cdiff_tuning *t);
+/* Replace the solver's tuning (any time between steps; CDIFF_BAD_ARG for a
+ * value out of range, the tuning then unchanged). */
+int cdiff_set_tuning(cdiff_solver *h, const cdiff_tuning *t);
+int cdiff_get_tuning(const cdiff_solver *h, cdiff_tuning *t);
+
 /* Create a solver for a mesh of `ndim` (2 or 3) with full (halo-inclusive)
  * dimensions dims_full[0..ndim-1] and mesh halo `nhalo` (>= 1).  bcs[2*ndim] in
  * (ilo, ihi, jlo, jhi, klo, khi) order -- the NamedTuple position of the

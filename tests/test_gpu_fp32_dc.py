@@ -37,10 +37,9 @@ def _oracle_residual_rms(mesh, bcs_o, kappa, T_new, T_old, dt):
     [("config1", "1"), ("config2", "1"), ("config1", "0"), ("config2", "0"), ("config4", "1"), ("config4", "0"),
      ("config4d", "1")],
 )
-def test_dc_reaches_fp64_tolerance(case, lin, monkeypatch):
+def test_dc_reaches_fp64_tolerance(case, lin):
     # lin = 1: the 2D correction sweep in coefficient form (several iterations
     # per launch); lin = 0: the conductivity-form sweep
-    monkeypatch.setenv("CDIFF_DC_LIN", lin)
     atol = 1e-8
     if case == "config1":
         mesh, T0 = gaussian_2d(96, ni=100)
@@ -65,7 +64,7 @@ def test_dc_reaches_fp64_tolerance(case, lin, monkeypatch):
               ("dirichlet", 1.0)]
         kappa, dt, nsteps = (1.0, 2.0, 0.0), 2e-4, 2
     law = S.PowerLawConductivity(*kappa)
-    dev = S.PseudoTransientSolver(mesh, bd)
+    dev = S.PseudoTransientSolver(mesh, bd, tuning={"dc_coefficient_form": int(lin)})
     ora = O.OracleSolver(mesh, bo, kappa=kappa)
     Td, To = T0.copy(order="F"), T0.copy(order="F")
     S.update_conductivity(dev, mesh, Td, 1.0, 1.0, law)
@@ -98,18 +97,16 @@ def _mesh_3d_dirichlet():
 
 
 @pytest.mark.parametrize("ch", ["32", "4", "1"])
-def test_fused_3d_outer_pass_matches_separate_kernels(ch, monkeypatch):
+def test_fused_3d_outer_pass_matches_separate_kernels(ch):
     # the fused 3D outer pass (one k-marching kernel: update, BC ghosts, alpha,
     # FP64 residual, sweep coefficients) against the separate per-cell kernels
     # it replaces; ch = planes per column chunk (11 planes: 1 chunk, 3 ragged
     # chunks, 11 chunks of one plane)
     mesh, T0, bd = _mesh_3d_dirichlet()
     law = S.PowerLawConductivity(1.0, 2.0, 0.0)
-    monkeypatch.setenv("CDIFF_OUT3_CH", ch)
     out = {}
     for fused in ("0", "1"):
-        monkeypatch.setenv("CDIFF_DC_FUSED3", fused)
-        dev = S.PseudoTransientSolver(mesh, bd)
+        dev = S.PseudoTransientSolver(mesh, bd, tuning={"dc_outer_planes3d": int(ch), "dc_fused3d": int(fused)})
         T = T0.copy(order="F")
         S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
         stats = []
@@ -149,7 +146,7 @@ def test_dc_small_ragged_meshes(ni, nj, neg):
 
 
 @pytest.mark.parametrize("ch,chk", [("11", "2"), ("4", "3"), ("1", "4"), ("3", "5")])
-def test_lin3_two_iteration_launch_matches_one(ch, chk, monkeypatch):
+def test_lin3_two_iteration_launch_matches_one(ch, chk):
     # the 3D correction sweep with two iterations per launch (two-cell ring,
     # k-marching pipeline) against one iteration per launch: the same FP32
     # expressions in the same order, so the same bits.  70 x 13 x 11 cells:
@@ -158,12 +155,10 @@ def test_lin3_two_iteration_launch_matches_one(ch, chk, monkeypatch):
     # values mix one- and two-iteration launches, and checks land on both)
     mesh, T0, bd = _mesh_3d_dirichlet()
     law = S.PowerLawConductivity(1.0, 2.0, 0.0)
-    monkeypatch.setenv("CDIFF_LIN3B_CH", ch)
-    monkeypatch.setenv("CDIFF_DC_CHECK", chk)
     out = {}
     for nit in ("2", "1"):
-        monkeypatch.setenv("CDIFF_LIN_NIT", nit)
-        dev = S.PseudoTransientSolver(mesh, bd)
+        dev = S.PseudoTransientSolver(mesh, bd, tuning={"dc_planes3d_pair": int(ch), "dc_check": int(chk),
+                                                         "dc_launch_iters": int(nit)})
         T = T0.copy(order="F")
         S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
         stats = []
@@ -193,18 +188,16 @@ def test_dc_then_fp64_step_continues():
 
 
 @pytest.mark.parametrize("depth", ["2", "4", "6", "8"])
-def test_lin_sweep_depth_does_not_change_result(depth, monkeypatch):
+def test_lin_sweep_depth_does_not_change_result(depth):
     # the coefficient-form sweep runs up to `depth` iterations per launch on a
     # tile with a depth-cell ring; per-cell arithmetic does not depend on how
     # the iterations are grouped, so one iteration per launch gives the same bits
     mesh, T0 = hot_disk_2d(150, seed=4)
     bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.FixedNegSymmetryBC(), S.NeumannBC()]
     law = S.PowerLawConductivity(1.0, 2.5, 0.0)
-    monkeypatch.setenv("CDIFF_LIN_K", depth)
     out = []
     for nit in (depth, "1"):
-        monkeypatch.setenv("CDIFF_LIN_NIT", nit)
-        dev = S.PseudoTransientSolver(mesh, bcs)
+        dev = S.PseudoTransientSolver(mesh, bcs, tuning={"dc_depth2d": int(depth), "dc_launch_iters": int(nit)})
         T = T0.copy(order="F")
         S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
         its = []
@@ -222,7 +215,7 @@ def test_lin_sweep_depth_does_not_change_result(depth, monkeypatch):
 @pytest.mark.parametrize("layout,depth,rows", [("1", "2", ""), ("1", "4", ""), ("1", "6", ""), ("1", "8", ""),
                                                ("2", "4", ""), ("2", "6", ""), ("2", "8", ""), ("2", "6", "17")])
 @pytest.mark.parametrize("n", [150, 61])
-def test_lin_sweep_row_layout_same_bits(layout, depth, rows, n, monkeypatch):
+def test_lin_sweep_row_layout_same_bits(layout, depth, rows, n):
     # the wave-aligned sweep (layout 1: tiles of 64 - 2 depth columns, region
     # rows on wavefronts) and the marching sweep (layout 2: one wavefront per
     # strip of 64 - 2 depth columns and chunk of rows, the iterations
@@ -233,13 +226,10 @@ def test_lin_sweep_row_layout_same_bits(layout, depth, rows, n, monkeypatch):
     bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.FixedNegSymmetryBC(), S.NeumannBC()]
     law = S.PowerLawConductivity(1.0, 2.5, 0.0)
     out = []
-    monkeypatch.setenv("CDIFF_DC_CHECK", "12")
-    monkeypatch.setenv("CDIFF_MARCH_ROWS", rows)
     for lay, k, nit in ((layout, depth, depth), ("0", "4", "1")):
-        monkeypatch.setenv("CDIFF_LIN2_LAYOUT", lay)
-        monkeypatch.setenv("CDIFF_LIN_K", k)
-        monkeypatch.setenv("CDIFF_LIN_NIT", nit)
-        dev = S.PseudoTransientSolver(mesh, bcs)
+        dev = S.PseudoTransientSolver(mesh, bcs, tuning={"dc_check": 12, "dc_march_rows": int(rows or 0),
+                                                          "dc_layout2d": int(lay), "dc_depth2d": int(k),
+                                                          "dc_launch_iters": int(nit)})
         T = T0.copy(order="F")
         S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
         its = []
@@ -287,7 +277,7 @@ def test_c3_mixed_precision_vs_implicit_direct_512():
     dev.close()
 
 
-def test_unstable_pseudo_time_step_falls_back(monkeypatch):
+def test_unstable_pseudo_time_step_falls_back():
     # an enlarged pseudo-time step that makes the correction sweeps diverge is
     # detected at the next inner check, dropped, and the step completes with the
     # reference's step to the same FP64 tolerance
@@ -296,8 +286,7 @@ def test_unstable_pseudo_time_step_falls_back(monkeypatch):
     law = S.PowerLawConductivity(1.0, 2.5, 0.0)
     res = {}
     for scale in ("1.0", "3.0"):
-        monkeypatch.setenv("CDIFF_DC_CFL_SCALE", scale)
-        dev = S.PseudoTransientSolver(mesh, bcs)
+        dev = S.PseudoTransientSolver(mesh, bcs, tuning={"dc_cfl_scale": float(scale)})
         T = T0.copy(order="F")
         S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
         for _ in range(2):

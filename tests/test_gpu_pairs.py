@@ -2,13 +2,12 @@
 
 The 2D sweep may run iterations t and t+1 in one kernel launch; its per-cell
 arithmetic is the single-iteration kernel's, so every field after a step must
-be bit-identical with the pairing switched off (CDIFF_PAIRS=0).  Covered: the
+be bit-identical with the pairing switched off (tuning pairs = 0).  Covered: the
 boundary-condition kinds that allow pairing, ragged tiles (sizes not multiples
 of the 64 x 8 tile, domains narrower than a tile), frozen conductivity with
 density and source, the manufactured-solution forcing, and the FP32 sweeps of
 the defect-correction path (with and without the Newton term).
 """
-import os
 
 import numpy as np
 import pytest
@@ -20,38 +19,30 @@ from curvilineardiffusionmanuscript_amd import twilight_zone as tz
 pytestmark = pytest.mark.gpu
 
 
-def _run(pairs, mesh, T0, bcs, law, nsteps, dt, *, rho=1.0, source=None, forcing=None, mean="harmonic", **opts):
-    old = os.environ.get("CDIFF_PAIRS")
-    os.environ["CDIFF_PAIRS"] = "1" if pairs else "0"
-    try:
-        dev = S.PseudoTransientSolver(mesh, bcs, mean=mean)
-        if source is not None:
-            dev.source_term = source
-        if forcing is not None:
-            dev.set_forcing(*forcing)
-        T = T0.copy(order="F")
-        S.update_conductivity(dev, mesh, T, rho, 1.0, law)
-        stats = []
-        for _ in range(nsteps):
-            st, nd = S.nonlinear_thermal_conduction_step(dev, mesh, T, rho, 1.0, law, dt, show_convergence=False,
-                                                         **opts)
-            stats.append((st.niter, st.abs_err, st.rel_err, nd))
-        fields = {
-            "T": T.copy(),
-            "alpha": dev.alpha,
-            "res": dev.res,
-            "q0": dev.q[0],
-            "q1": dev.q[1],
-            "qp0": dev.q_prime[0],
-            "qp1": dev.q_prime[1],
-        }
-        dev.close()
-        return stats, fields
-    finally:
-        if old is None:
-            os.environ.pop("CDIFF_PAIRS", None)
-        else:
-            os.environ["CDIFF_PAIRS"] = old
+def _run(pairs, mesh, T0, bcs, law, nsteps, dt, *, rho=1.0, source=None, forcing=None, mean="harmonic", tuning=None,
+         **opts):
+    dev = S.PseudoTransientSolver(mesh, bcs, mean=mean, tuning=dict(tuning or {}, pairs=int(pairs)))
+    if source is not None:
+        dev.source_term = source
+    if forcing is not None:
+        dev.set_forcing(*forcing)
+    T = T0.copy(order="F")
+    S.update_conductivity(dev, mesh, T, rho, 1.0, law)
+    stats = []
+    for _ in range(nsteps):
+        st, nd = S.nonlinear_thermal_conduction_step(dev, mesh, T, rho, 1.0, law, dt, show_convergence=False, **opts)
+        stats.append((st.niter, st.abs_err, st.rel_err, nd))
+    fields = {
+        "T": T.copy(),
+        "alpha": dev.alpha,
+        "res": dev.res,
+        "q0": dev.q[0],
+        "q1": dev.q[1],
+        "qp0": dev.q_prime[0],
+        "qp1": dev.q_prime[1],
+    }
+    dev.close()
+    return stats, fields
 
 
 def _same(*args, fp32=False, **kw):
@@ -110,10 +101,10 @@ def test_pairs_bit_identical_forcing():
 
 
 @pytest.mark.parametrize("newton", ["0", "1"])
-def test_pairs_bit_identical_fp32_dc(newton, monkeypatch):
-    monkeypatch.setenv("CDIFF_DC_NEWTON", newton)
-    monkeypatch.setenv("CDIFF_DC_LIN", "0")  # the conductivity-form FP32 sweep
+def test_pairs_bit_identical_fp32_dc(newton):
+    # the conductivity-form FP32 sweep (dc_coefficient_form = 0)
     mesh, T0 = hot_disk_2d(96, seed=0)
     bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
     _same(mesh, T0, bcs, S.PowerLawConductivity(1.0, 2.5, 0.0), 3, 1e-5, fp32=True, rtol=0.0, atol=1e-8,
-          max_iter=10**6, precision="fp32_dc", calculate_next_dt=False)
+          max_iter=10**6, precision="fp32_dc", calculate_next_dt=False,
+          tuning={"dc_newton": int(newton), "dc_coefficient_form": 0})

@@ -35,10 +35,10 @@ def _bcs(spec):
 
 
 def _run_pair(mesh, T0, bcs_spec, kappa, nsteps, dt0, *, rho=None, source=None, mean="harmonic", adaptive=False,
-              cp=1.0, **opts):
+              cp=1.0, tuning=None, **opts):
     bd, bo = _bcs(bcs_spec)
     law = S.PowerLawConductivity(*kappa)
-    dev = S.PseudoTransientSolver(mesh, bd, mean=mean)
+    dev = S.PseudoTransientSolver(mesh, bd, mean=mean, tuning=tuning)
     ora = O.OracleSolver(mesh, bo, mean=mean, kappa=kappa, cp=cp, rho=rho, source=source)
     if source is not None:
         dev.source_term = source
@@ -93,6 +93,38 @@ def test_config1_gaussian_multi_strip():
     _, _, out = _run_pair(mesh, T0, ["neumann"] * 4, (1.0, 0.0, 0.0), 4, 1e-4, rtol=0.0, atol=1e-8,
                           max_iter=100000, calculate_next_dt=False)
     _assert_parity(out, rel=0.0)  # kappa = 1: no pow(), the sweep is bit-exact
+
+
+@pytest.mark.parametrize("tune", [{"sweep2d": 1}, {"tile_rows": 4, "pairs": 0}, {"tile_rows": 8, "pairs": 0},
+                                  {"tile_rows": 16, "pairs": 0}])
+def test_fp64_sweep_variants_match_the_oracle(tune):
+    # the row-marching strips (sweep2d = 1: one wavefront per strip of 62
+    # columns, DPP shifts along the row) and the LDS tiles of each height,
+    # one iteration per launch, give the oracle's bits: kappa = 1 (no pow()),
+    # 70 x 45 cells with ragged strips; then kappa = T^2.5 with a Dirichlet and
+    # an odd-symmetry face on the adaptive hot disk
+    mesh, T0 = gaussian_2d(45, ni=70)
+    _, _, out = _run_pair(mesh, T0, ["neumann"] * 4, (1.0, 0.0, 0.0), 3, 1e-4, rtol=0.0, atol=1e-8,
+                          max_iter=100000, calculate_next_dt=False, tuning=tune)
+    _assert_parity(out, rel=0.0)
+    mesh, T0 = hot_disk_2d(40, seed=3)
+    _, _, out = _run_pair(mesh, T0, [("dirichlet", 1.0), "neumann", "neg", "neumann"], (1.0, 2.5, 0.0), 3, 1e-5,
+                          rtol=0.0, atol=1e-8, max_iter=100000, adaptive=True, tuning=tune)
+    _assert_parity(out, err_rel=1e-3)  # T^2.5: pow() against x*x*sqrt(x), as in the config-2 test
+
+
+def test_tuning_round_trip_and_range_checks():
+    mesh, _ = gaussian_2d(32)
+    dev = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4, tuning={"dc_eta": 2e-3, "verbose": 0})
+    t = dev.get_tuning()
+    assert t["dc_eta"] == 2e-3 and t["dc_depth2d"] == 6 and t["tile_rows"] == 8 and t["dc_layout2d"] == 2
+    for bad in ({"tile_rows": 5}, {"dc_depth2d": 3}, {"dc_eta": 1.0}, {"seg_min": 0}, {"dc_cfl_scale": 0.0}):
+        with pytest.raises(S.SolverAssertionError, match="cdiff_set_tuning"):
+            dev.set_tuning(**bad)
+        assert dev.get_tuning() == t  # unchanged
+    with pytest.raises(TypeError):
+        dev.set_tuning(no_such_field=1)
+    dev.close()
 
 
 def test_config2_hot_disk_nonlinear_dirichlet_adaptive():

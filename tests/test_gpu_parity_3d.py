@@ -15,19 +15,23 @@ pytestmark = pytest.mark.gpu
 _DEV = {"neumann": S.NeumannBC(), "periodic": S.PeriodicBC()}
 
 
+_TUNE = {}
+
+
 @pytest.fixture(params=["col", "lanes"], autouse=True)
-def sweep3d_kernel(request, monkeypatch):
+def sweep3d_kernel(request):
     # col: 64 x 8 columns with one-cell rings (default); lanes: one lane per
     # column with lane/wave rings.  Both must reproduce the oracle bit for bit.
-    monkeypatch.setenv("CDIFF_SWEEP3D", request.param)
-    return request.param
+    _TUNE["sweep3d"] = 0 if request.param == "col" else 1
+    yield request.param
+    _TUNE.clear()
 
 
 def _pair(mesh, T0, spec, kappa, nsteps, dt, **opts):
     bd = [S.DirichletBC(s[1]) if isinstance(s, tuple) else _DEV[s] for s in spec]
     bo = [("dirichlet", s[1]) if isinstance(s, tuple) else (s, 0.0) for s in spec]
     law = S.PowerLawConductivity(*kappa)
-    dev = S.PseudoTransientSolver(mesh, bd)
+    dev = S.PseudoTransientSolver(mesh, bd, tuning=_TUNE)
     ora = O.OracleSolver(mesh, bo, kappa=kappa)
     Td, To = T0.copy(order="F"), T0.copy(order="F")
     S.update_conductivity(dev, mesh, Td, 1.0, 1.0, law)
