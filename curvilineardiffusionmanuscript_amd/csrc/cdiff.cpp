@@ -205,6 +205,7 @@ void cdiff_default_tuning(cdiff_tuning* t) {
   t->dc_cfl_scale = 1.2;
   t->dc_dtau_scale = 1.3;
   t->dc_theta_scale = 1.0;
+  t->graphs = 1;
 }
 
 int cdiff_set_tuning(cdiff_solver* h, const cdiff_tuning* t) {
@@ -233,6 +234,7 @@ int cdiff_set_tuning(cdiff_solver* h, const cdiff_tuning* t) {
     set_law(h, &k, h->cp);
   }
   // the FP64 launch geometry and the correction parameters are chosen again
+  h->graph_reset();
   h->configure_sweep();
   int rc = h->ensure_parts();
   if (rc) return rc;
@@ -326,6 +328,9 @@ int cdiff_create(int ndim, const int64_t* dims_full, int nhalo, int mean, const 
   for (int e = 0; e < 2; ++e) HIP_TRY(hipEventCreateWithFlags(&h->ev[e], hipEventDisableTiming));
   h->configure_sweep();
   if ((rc = h->ensure_parts())) return rc;
+  HIP_TRY(hipMalloc((void**)&h->ic_dev, sizeof(IterConst)));
+  h->owned.push_back(h->ic_dev);
+  HIP_TRY(hipHostMalloc((void**)&h->ic_host, sizeof(IterConst), hipHostMallocDefault));
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0.0, 0.0);
   LAUNCH_CHECK("k_ctrl_reset");
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -336,7 +341,9 @@ int cdiff_create(int ndim, const int64_t* dims_full, int nhalo, int mean, const 
 int cdiff_destroy(cdiff_solver* h) {
   if (!h) return CDIFF_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->graph_reset();
   for (void* p : h->owned) (void)hipFree(p);
+  if (h->ic_host) (void)hipHostFree(h->ic_host);
   for (void* p : h->owned32) (void)hipFree(p);
   if (h->ctrl_h) (void)hipHostFree(h->ctrl_h);
   if (h->red_h) (void)hipHostFree(h->red_h);

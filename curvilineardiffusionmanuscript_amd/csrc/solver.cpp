@@ -164,44 +164,50 @@ bool cdiff_solver::pairs_ok() const {
   return true;
 }
 
-int cdiff_solver::launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check, bool pair) {
+Sweep2DArgs<double> cdiff_solver::sweep2d_args(const cdiff_opts& o, double dt, double Vpdtau, int src) const {
   const int dst = 1 - src;
+  Sweep2DArgs<double> A{};
+  A.g = g;
+  A.gh = gh;
+  A.kap = kap;
+  A.ic.L = L;
+  A.ic.Vpdtau = Vpdtau;
+  A.ic.dt = dt;
+  A.harmonic = mean == CDIFF_MEAN_HARMONIC;
+  A.ntiles = ntiles;
+  A.chunks = chunks;
+  A.ch = ch;
+  A.u_src = u[src];
+  A.u_dst = u[dst];
+  A.qx_src = q[0][src];
+  A.qx_dst = q[0][dst];
+  A.qy_src = q[1][src];
+  A.qy_dst = q[1][dst];
+  A.u_prev = u_prev;
+  A.g11 = coef[0];
+  A.g22 = coef[1];
+  A.c12 = coef[2];
+  A.ac = coef[3];
+  A.bc = coef[4];
+  A.S = S;
+  A.rho = rho;
+  A.tzA = tzA;
+  A.tzB = tzB;
+  if (!o.subcycle_conductivity) {
+    A.frozen_alpha = frozen[0];
+    A.frozen_theta = frozen[1];
+    A.frozen_dtau = frozen[2];
+  }
+  A.part_sum = part_sum;
+  A.part_max = part_max;
+  A.ctrl = ctrl;
+  return A;
+}
+
+int cdiff_solver::launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check, bool pair) {
   if (ndim == 2) {
-    Sweep2DArgs<double> A{};
-    A.g = g;
-    A.gh = gh;
-    A.kap = kap;
-    A.ic.L = L;
-    A.ic.Vpdtau = Vpdtau;
-    A.ic.dt = dt;
-    A.harmonic = mean == CDIFF_MEAN_HARMONIC;
-    A.ntiles = ntiles;
-    A.chunks = chunks;
-    A.ch = ch;
-    A.u_src = u[src];
-    A.u_dst = u[dst];
-    A.qx_src = q[0][src];
-    A.qx_dst = q[0][dst];
-    A.qy_src = q[1][src];
-    A.qy_dst = q[1][dst];
-    A.u_prev = u_prev;
-    A.g11 = coef[0];
-    A.g22 = coef[1];
-    A.c12 = coef[2];
-    A.ac = coef[3];
-    A.bc = coef[4];
-    A.S = S;
-    A.rho = rho;
-    A.tzA = tzA;
-    A.tzB = tzB;
-    if (!o.subcycle_conductivity) {
-      A.frozen_alpha = frozen[0];
-      A.frozen_theta = frozen[1];
-      A.frozen_dtau = frozen[2];
-    }
-    A.part_sum = part_sum;
-    A.part_max = part_max;
-    A.ctrl = ctrl;
+    Sweep2DArgs<double> A = sweep2d_args(o, dt, Vpdtau, src);
+    if (graph_capture) A.ic_dev = ic_dev;
     if (pair) {
       A.u_pre = check ? u_pre : nullptr;
       launch_sweep2d_pair<double>(A, check, nblocks, stream);
@@ -213,6 +219,7 @@ int cdiff_solver::launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, 
     LAUNCH_CHECK("sweep2d");
     return CDIFF_OK;
   }
+  const int dst = 1 - src;
   Sweep3DArgs<double> A{};
   A.g = g;
   A.gh = gh;
@@ -271,6 +278,68 @@ int cdiff_solver::enqueue_next_dt(const double* unew, const double* uold, double
   return CDIFF_OK;
 }
 
+bool cdiff_solver::graph_ok() const { return tune.graphs && ndim == 2 && tile2d && ic_dev; }
+
+void cdiff_solver::graph_reset() {
+  for (auto& b : gblock) {
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    b.exec = nullptr;
+    b.key.clear();
+  }
+}
+
+// GRAPH_ITERS pseudo-iterations without a check from buffer `src` (an even
+// number of launches, so the block ends on the parity it started from): the
+// cached graph when it was captured from the same launches, else a new
+// capture.  The kernels read this step's dt and Vpdtau from ic_dev.
+int cdiff_solver::graph_block(const cdiff_opts& o, double dt, double Vpdtau, int src, bool pair) {
+  GraphBlock& gb = gblock[src];
+  // everything the captured launches bake in besides the iteration constants:
+  // buffers, optional arrays, the conductivity law and the launch shape (the
+  // geometry and ghost rules are fixed for the handle)
+  const Sweep2DArgs<double> A = sweep2d_args(o, 0.0, 0.0, src);
+  auto bits = [](double v) {
+    uint64_t b;
+    std::memcpy(&b, &v, sizeof b);
+    return b;
+  };
+  auto ptr = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+  const std::vector<uint64_t> key = {
+      ptr(A.u_src), ptr(A.u_dst), ptr(A.qx_src), ptr(A.qx_dst), ptr(A.qy_src), ptr(A.qy_dst), ptr(A.u_prev),
+      ptr(A.g11), ptr(A.g22), ptr(A.c12), ptr(A.ac), ptr(A.bc), ptr(A.S), ptr(A.rho), ptr(A.tzA), ptr(A.tzB),
+      ptr(A.frozen_alpha), ptr(A.frozen_theta), ptr(A.frozen_dtau), ptr(A.part_sum), ptr(A.part_max),
+      ptr(A.ctrl), ptr(ic_dev), bits(kap.kappa0), bits(kap.t_exp), bits(kap.rho_exp), bits(kap.cp),
+      (uint64_t)kap.t_int, (uint64_t)kap.rho_int, (uint64_t)kap.t_half, (uint64_t)A.harmonic,
+      (uint64_t)A.ntiles, (uint64_t)A.chunks, (uint64_t)A.ch, (uint64_t)nblocks, (uint64_t)pair,
+      (uint64_t)tile2d};
+  if (!gb.exec || gb.key != key) {
+    if (gb.exec) (void)hipGraphExecDestroy(gb.exec);
+    gb.exec = nullptr;
+    gb.key.clear();
+    const int nlaunch = pair ? GRAPH_ITERS / 2 : GRAPH_ITERS;
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    graph_capture = true;
+    int rc = CDIFF_OK, b = src;
+    for (int l = 0; l < nlaunch && rc == CDIFF_OK; ++l, b ^= 1) rc = launch_sweep64(o, dt, Vpdtau, b, false, pair);
+    graph_capture = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &graph);
+    if (rc != CDIFF_OK || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc != CDIFF_OK ? rc : hip_fail(e, "hipStreamEndCapture");
+    }
+    const hipError_t ei = hipGraphInstantiate(&gb.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      gb.exec = nullptr;
+      return hip_fail(ei, "hipGraphInstantiate");
+    }
+    gb.key = key;
+  }
+  HIP_TRY(hipGraphLaunch(gb.exec, stream));
+  return CDIFF_OK;
+}
+
 // Launch the FP64 sweeps of one time step in segments (see cdiff.cpp).
 int cdiff_solver::run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int /*first_iter_base*/) {
   int m, k;
@@ -289,12 +358,29 @@ int cdiff_solver::run_loop64(const cdiff_opts& o, double dt, double Vpdtau, int 
   int buf = cur;  // source buffer of the next launch
   bool pending = false;
   launches.clear();
+  const bool graphs = graph_ok();
+  if (graphs) {
+    *ic_host = IterConst{L, Vpdtau, dt};
+    HIP_TRY(hipMemcpyAsync(ic_dev, ic_host, sizeof(IterConst), hipMemcpyHostToDevice, stream));
+  }
   while (true) {
     const int64_t s = it + 1;
     const int64_t cap = std::min<int64_t>(s + lmax - 1, limit);
     int64_t e = std::min<int64_t>(s + lmin - 1, cap);
     while (e < cap && !is_check((int)e, m, k)) ++e;
     for (int64_t t = s; t <= e;) {
+      if (graphs && t + GRAPH_ITERS - 1 <= e) {
+        bool clear = true;
+        for (int64_t x = t; x < t + GRAPH_ITERS && clear; ++x) clear = !is_check((int)x, m, k);
+        if (clear) {
+          int rc = graph_block(o, dt, Vpdtau, buf, pairs);
+          if (rc) return rc;
+          for (int l = 0, b = buf; l < (pairs ? GRAPH_ITERS / 2 : GRAPH_ITERS); ++l, b ^= 1)
+            launches.push_back({(int)(t - 1 + (pairs ? 2 * (l + 1) : l + 1)), 1 - b, pairs});
+          t += GRAPH_ITERS;
+          continue;
+        }
+      }
       const bool chk = is_check((int)t, m, k);
       const bool pair = pairs && !chk && t + 1 <= e;
       const int64_t tl = pair ? t + 1 : t;  // last iteration of this launch

@@ -7,6 +7,7 @@
 
 #include "../../include/cdiff.h"
 #include "cdiff_dev.h"
+#include "sweep_args.h"
 
 struct cdiff_solver {
   int device = 0;
@@ -113,6 +114,21 @@ struct cdiff_solver {
   int enforce_pos_last = 0;
   bool t_written = false;
 
+  // graph replay of blocks of FP64 2D pseudo-iterations without a check
+  // (tune.graphs): the launches of GRAPH_ITERS iterations from each source
+  // buffer parity, captured once and replayed while their arguments (this
+  // step's iteration constants come from ic_dev) stay the same
+  static constexpr int GRAPH_ITERS = 16;
+  cdiff::IterConst* ic_dev = nullptr;   // device copy of the step's iteration constants
+  cdiff::IterConst* ic_host = nullptr;  // its pinned staging
+  bool graph_capture = false;           // launch_sweep64 is being captured
+  struct GraphBlock {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> key;  // the arguments and launch shape of the capture
+  };
+  GraphBlock gblock[2];  // by the source buffer of the block's first launch
+  void graph_reset();
+
   struct Launch {
     int last_iter;  // last pseudo-iteration of the launch
     int dst;        // buffer it wrote
@@ -155,6 +171,9 @@ struct cdiff_solver {
   int validation_message(int status);
   int step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cdiff_stats* stats);
   int launch_sweep64(const cdiff_opts& o, double dt, double Vpdtau, int src, bool check, bool pair = false);
+  cdiff::Sweep2DArgs<double> sweep2d_args(const cdiff_opts& o, double dt, double Vpdtau, int src) const;
+  bool graph_ok() const;
+  int graph_block(const cdiff_opts& o, double dt, double Vpdtau, int src, bool pair);
   int launch_sweep32(double dt, double Vpdtau, int src, bool check, bool pair = false);
   int launch_lin(double dt, int src, bool check, int nit, bool zero_src = false);
   int lin_coefs(double dt, double Vpdtau, int buf);

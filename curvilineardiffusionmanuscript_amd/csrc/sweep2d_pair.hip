@@ -100,6 +100,7 @@ __device__ __forceinline__ bool c1_cell(int slot, int tid, int& lx, int& ly) {
 
 template <typename T, bool CHECK, int TY>
 __global__ __launch_bounds__(PTX* TY) void sweep2d_pair_kernel(Sweep2DArgs<T> A) {
+  const IterConst ic = A.ic_dev ? *A.ic_dev : A.ic;
   constexpr int LY = TY + 4;
   constexpr int NC2 = LX * LY;
   constexpr int NTHR = PTX * TY;
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(PTX* TY) void sweep2d_pair_kernel(Sweep2DArgs<T> A)
   const int x0 = g.lo[0] + bx * PTX, y0 = g.lo[1] + by * TY;
   const int gx = x0 - 2, gy = y0 - 2;  // global index of local (0, 0)
   const int harm = A.harmonic;
-  const T dt = (T)A.ic.dt;
+  const T dt = (T)ic.dt;
   const bool newton = F32 && A.ubar != nullptr;
 
   // cell (i, r) takes part iff it is a domain cell or a first-layer ghost
@@ -246,14 +247,14 @@ __global__ __launch_bounds__(PTX* TY) void sweep2d_pair_kernel(Sweep2DArgs<T> A)
       if (A.frozen_alpha) {
         a = lfa[p];
         if (A.alpha_only) {
-          iter_params<T>(A.ic, rho, a, dtau, th);
+          iter_params<T>(ic, rho, a, dtau, th);
         } else {
           th = lfth[p];
           dtau = lfdt[p];
         }
       } else {
         a = alpha_of<T>(A.kap, rho, lu[p]);
-        iter_params<T>(A.ic, rho, a, dtau, th);
+        iter_params<T>(ic, rho, a, dtau, th);
       }
     }
     sU0[t] = lu[p];
@@ -364,14 +365,14 @@ __global__ __launch_bounds__(PTX* TY) void sweep2d_pair_kernel(Sweep2DArgs<T> A)
     if (A.frozen_alpha) {
       a = sA0[f];  // frozen: unchanged by the iteration
       if (A.alpha_only) {
-        iter_params<T>(A.ic, rho, a, dtau, th);
+        iter_params<T>(ic, rho, a, dtau, th);
       } else {
         th = sT0[f];
         dtau = sD0[f];
       }
     } else {
       a = alpha_of<T>(A.kap, rho, v);
-      iter_params<T>(A.ic, rho, a, dtau, th);
+      iter_params<T>(ic, rho, a, dtau, th);
     }
     a1v[p] = a;
     t1v[p] = th;

@@ -72,6 +72,7 @@ __device__ __forceinline__ bool approx_eps(T x, T y) {
 
 template <typename T, bool CHECK, int TY>
 __global__ __launch_bounds__(TX* TY) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
+  const IterConst ic = A.ic_dev ? *A.ic_dev : A.ic;
   constexpr int CY = TY + 2;             // ringed rows
   constexpr int NCELL = CX * CY;         // cells with ring
   constexpr int NFX = FXN * CY;          // xi faces
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(TX* TY) void sweep2d_tile_kernel(Sweep2DArgs<T> A) 
   const int bx = tile / ty_n, by = tile - bx * ty_n;
   const int x0 = g.lo[0] + bx * TX, y0 = g.lo[1] + by * TY;
   const int harm = A.harmonic;
-  const T dt = (T)A.ic.dt;
+  const T dt = (T)ic.dt;
 
   // ---- all global loads up front: one memory round trip per block
   constexpr int P1 = (NCELL + NTHR - 1) / NTHR;        // 2 cells per thread
@@ -182,14 +183,14 @@ __global__ __launch_bounds__(TX* TY) void sweep2d_tile_kernel(Sweep2DArgs<T> A) 
       if (A.frozen_alpha) {
         a = lfa[p];
         if (A.alpha_only) {
-          iter_params<T>(A.ic, rho, a, dtau, th);
+          iter_params<T>(ic, rho, a, dtau, th);
         } else {
           th = lfth[p];
           dtau = lfdt[p];
         }
       } else {
         a = alpha_of<T>(A.kap, rho, u);
-        iter_params<T>(A.ic, rho, a, dtau, th);
+        iter_params<T>(ic, rho, a, dtau, th);
       }
       // first-layer ghosts of this tile's owned rows/columns go to the new iterate
       const bool gi = (i == g.lo[0] - 1) | (i == g.hi[0] + 1);

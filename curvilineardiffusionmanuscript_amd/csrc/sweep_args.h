@@ -10,6 +10,9 @@ struct Sweep2DArgs {
   Ghost gh;
   KappaLaw kap;
   IterConst ic;
+  // nullable: the iteration constants in device memory instead of `ic`, so
+  // that one captured graph of launches serves every time step
+  const IterConst* ic_dev;
   int harmonic;
   int ntiles, chunks, ch;
   const T* __restrict__ u_src;

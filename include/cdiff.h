@@ -179,6 +179,8 @@ typedef struct {
   int32_t dc_max_outer;     /* correction rounds before the step is redone by the FP64 APT: 100 */
   int32_t verbose;          /* 1: per-round diagnostics of the defect correction on stderr */
   int32_t exact_pow;        /* 1: pow() for T^(n+1/2) instead of T^n sqrt(T) (within 2 ulp) */
+  int32_t graphs;           /* 1: 2D FP64 blocks of 16 pseudo-iterations without a check replayed
+                               from captured HIP graphs (default), 0: launched one by one */
   double dc_eta;            /* inner solves stop at eta x the outer residual; 0 = auto (1e-3 2D, 1e-2 3D) */
   double dc_floor;          /* ... or at floor x atol: 0.3 */
   double dc_cfl_scale;      /* pseudo-time CFL of the correction sweeps over the step's: 1.2 */

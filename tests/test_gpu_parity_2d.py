@@ -113,6 +113,48 @@ def test_fp64_sweep_variants_match_the_oracle(tune):
     _assert_parity(out, err_rel=1e-3)  # T^2.5: pow() against x*x*sqrt(x), as in the config-2 test
 
 
+@pytest.mark.parametrize("n,adaptive,kappa,pairs", [(128, False, (1.0, 0.0, 0.0), 1), (96, True, (1.0, 2.5, 0.0), 1),
+                                                   (96, True, (1.0, 2.5, 0.0), 0)])
+def test_graph_blocks_bit_identical(n, adaptive, kappa, pairs):
+    # blocks of 16 pseudo-iterations without a check replayed from captured
+    # HIP graphs (the default) against the same launches made one by one:
+    # every step's temperature, fields, iteration count and next_dt bit for
+    # bit.  The graphs are replayed many times per step with eager launches
+    # (checks) and host synchronisations between the replays, and across
+    # steps whose dt changes (adaptive), which they read from device memory.
+    if adaptive:
+        mesh, T0 = gaussian_2d(n)
+        bd = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
+        dt0 = 2e-5
+    else:
+        mesh, T0 = gaussian_2d(n)
+        bd = [S.NeumannBC()] * 4
+        dt0 = 1e-4
+    law = S.PowerLawConductivity(*kappa)
+    res = {}
+    for graphs in (0, 1):
+        dev = S.PseudoTransientSolver(mesh, bd, tuning={"graphs": graphs, "pairs": pairs})
+        T = T0.copy(order="F")
+        S.update_conductivity(dev, mesh, T, 1.0, 1.0, law)
+        dt, out = dt0, []
+        for _ in range(6):
+            st, nd = S.nonlinear_thermal_conduction_step(dev, mesh, T, 1.0, 1.0, law, dt, rtol=0.0, atol=1e-8,
+                                                         max_iter=10**6, show_convergence=False)
+            out.append((st.niter, st.abs_err, nd, T.copy(), dev.q[0], dev.res))
+            if adaptive:
+                dt = nd
+        res[graphs] = out
+        dev.close()
+    # after the first step (checks every iteration) the check schedule leaves
+    # stretches of more than 16 iterations without a check: blocks replayed
+    print([o[0] for o in res[1]])
+    assert min(o[0] for o in res[1][1:]) >= 40
+    for a, b in zip(res[0], res[1]):
+        assert a[:3] == b[:3]
+        for x, y in zip(a[3:], b[3:]):
+            np.testing.assert_array_equal(x, y)
+
+
 def test_tuning_round_trip_and_range_checks():
     mesh, _ = gaussian_2d(32)
     dev = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4, tuning={"dc_eta": 2e-3, "verbose": 0})
