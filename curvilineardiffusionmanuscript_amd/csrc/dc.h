@@ -76,7 +76,6 @@ struct DcOuter3D {
   const double* u_old;
   const float* e;  // nullable: no correction
   double* u_new;
-  double* a64;
   const double* uprev;
   const double* c;  // g11 g22 g33 g12 g13 g23 ac bc gc: array q at c + q * cstride
   int64_t cstride;
@@ -88,7 +87,7 @@ struct DcOuter3D {
   double* psum;
   double* pmax;
 };
-__global__ void k_dc_outer3d(DcOuter3D A);
+void launch_dc_outer3d(const DcOuter3D& A, int nblk, bool unit, hipStream_t stream);
 __global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double t_exp, int newton, const double* rho,
                             const double* u, const double* a, LinOut out);
 __global__ void k_to_f32(int64_t n, const double* src, float* dst);

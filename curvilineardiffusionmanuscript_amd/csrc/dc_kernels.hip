@@ -14,6 +14,7 @@
 
 #include "cdiff_dev.h"
 #include "dc.h"
+#include "lin_coef_ops.h"
 
 namespace cdiff {
 
@@ -335,12 +336,6 @@ __device__ __forceinline__ double ghost_factor(const Ghost& gh, int f) {
   return m == G_COPY ? 1.0 : (m == G_NEG ? -1.0 : 0.0);
 }
 
-// d(alpha)/du of kappa0 |rho|^a |u|^b / (rho cp): b alpha / u (0 where undefined)
-__device__ __forceinline__ double dalpha_du(double t_exp, double a, double u) {
-  const double d = (u != 0.0 && isfinite(u)) ? t_exp * a / u : 0.0;
-  return isfinite(d) ? d : 0.0;
-}
-
 }  // namespace
 
 // Per face of expand_lower(domain, axis), from the FP64 iterate u and its
@@ -363,14 +358,14 @@ __global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double 
     dom = dom && idx[d] >= g.lo[d] && idx[d] <= g.hi[d];
     near = near && idx[d] >= g.lo[d] - 1 && idx[d] <= g.hi[d] + 1;
   }
-  auto params = [&](int64_t cell, double& dtau, double& theta) {
-    iter_params<double>(ic, rho ? rho[cell] : 1.0, a[cell], dtau, theta);
-    dtau *= out.dtau_scale;
-    theta *= out.theta_scale;
+  // the sweep's coefficients in FP32 (lin_coef_ops.h), as the fused outer passes form them
+  const IterConst32 k32 = iter_const32(ic, out.dtau_scale, out.theta_scale);
+  auto params = [&](int64_t cell, float& C, float& theta) {
+    iter_params32(k32, (float)(rho ? rho[cell] : 1.0), (float)a[cell], C, theta);
   };
-  double dtau0 = 0.0, th0 = 0.0;
-  if (near) params(c, dtau0, th0);
-  out.cc[c] = dom ? (float)(dtau0 / (1.0 + dtau0 / ic.dt)) : 0.f;
+  float C0 = 0.f, th0 = 0.f;
+  if (near) params(c, C0, th0);
+  out.cc[c] = dom ? C0 : 0.f;
   for (int ax = 0; ax < g.ndim; ++ax) {
     float* const* F = ax == 0 ? out.fx : (ax == 1 ? out.fy : out.fz);
     bool on = true;
@@ -383,34 +378,28 @@ __global__ void k_lin_coefs(Geo g, Ghost gh, int harmonic, IterConst ic, double 
       continue;
     }
     const int64_t c1 = c + (ax == 0 ? (int64_t)1 : (ax == 1 ? g.pitch : g.plane));
-    double dtau1, th1;
-    params(c1, dtau1, th1);
+    float C1, th1;
+    params(c1, C1, th1);
     const double a0 = a[c], a1 = a[c1];
-    const double abar = face_mean<double>(harmonic, a0, a1);
-    const double tbar = face_mean<double>(harmonic, th0, th1);
-    double P = abar, M = abar;
+    float P = (float)face_mean<double>(harmonic, a0, a1), M = P;
     if (newton) {
-      const double s = a0 + a1;
-      double d0 = 0.5, d1 = 0.5;  // derivatives of the face mean in a0, a1
-      if (harmonic) {
-        d0 = (s != 0.0) ? 2.0 * a1 * a1 / (s * s) : 0.0;
-        d1 = (s != 0.0) ? 2.0 * a0 * a0 / (s * s) : 0.0;
-      }
-      const double du = u[c1] - u[c];
-      P -= d0 * dalpha_du(t_exp, a0, u[c]) * du;
-      M += d1 * dalpha_du(t_exp, a1, u[c1]) * du;
+      float d0, d1;  // derivatives of the face mean in a0, a1
+      mean_weights32(harmonic, (float)a0, (float)a1, d0, d1);
+      const float du = (float)(u[c1] - u[c]);
+      P = __builtin_fmaf(-d0 * dadu32((float)t_exp, (float)a0, (float)u[c]), du, P);
+      M = __builtin_fmaf(d1 * dadu32((float)t_exp, (float)a1, (float)u[c1]), du, M);
     }
     // boundary faces: cell 0 below the domain, or cell 1 above it
     if (idx[ax] == g.lo[ax] - 1) {
-      M -= ghost_factor(gh, 2 * ax) * P;
-      P = 0.0;
+      M = __builtin_fmaf(-(float)ghost_factor(gh, 2 * ax), P, M);
+      P = 0.f;
     } else if (idx[ax] == g.hi[ax]) {
-      P -= ghost_factor(gh, 2 * ax + 1) * M;
-      M = 0.0;
+      P = __builtin_fmaf(-(float)ghost_factor(gh, 2 * ax + 1), M, P);
+      M = 0.f;
     }
-    F[0][c] = (float)P;
-    F[1][c] = (float)M;
-    F[2][c] = (float)(tbar / (1.0 + tbar));
+    F[0][c] = P;
+    F[1][c] = M;
+    F[2][c] = damping32(harmonic, th0, th1);
   }
 }
 

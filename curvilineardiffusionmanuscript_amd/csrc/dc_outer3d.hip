@@ -6,12 +6,14 @@
 // planes z0..z1.  Plane step p, for p = z0-1 .. z1+1:
 //   cells    of the ringed 66 x 10 plane p: u = u_old + e (domain), the BC
 //            rule for first-layer ghosts (local rules only: no periodic
-//            faces), alpha(u), theta, dtau, d(alpha)/du (FP64, LDS, planes p-1
-//            and p); writes u, alpha and the sweep's C of the column
+//            faces), alpha(u) in FP64; theta, the update coefficient C and
+//            d(alpha)/du in FP32 (lin_coef_ops.h) (LDS, planes p-1 and p);
+//            writes u and C of the column (alpha is not stored: the step's
+//            epilogue forms it from the final iterate)
 //   faces    q' = -abar (u1 - u0) on expand_lower(domain, axis) (FP64, LDS)
 //            of the xi and eta faces of plane p and of the zeta faces between
 //            planes p-1 and p; the sweep coefficients P, M, W of the column's
-//            faces (k_lin_coefs)
+//            faces in FP32, as k_lin_coefs forms them
 //   residual of plane p-1 per column cell, in FP64: the reference residual
 //            (residuals.jl:24-47) with the 3D divergence of
 //            flux_divergence.jl:119-239, summed in the order of k_dc_residual;
@@ -23,6 +25,7 @@
 
 #include "column3d.h"
 #include "dc.h"
+#include "lin_coef_ops.h"
 
 namespace cdiff {
 
@@ -33,14 +36,9 @@ constexpr int FYN = L3Y + 1;  // eta face rows: between region rows fy and fy+1
 constexpr int NFX = FXN * R3Y, NFY = R3X * FYN;
 constexpr int NFACE = NFX + NFY + R3;
 
-__device__ __forceinline__ double gfac3(const Ghost& gh, int f) {
+__device__ __forceinline__ float gfac3(const Ghost& gh, int f) {
   const int m = gh.mode[f];
-  return m == G_COPY ? 1.0 : (m == G_NEG ? -1.0 : 0.0);
-}
-
-__device__ __forceinline__ double dadu3(double t_exp, double a, double u) {
-  const double d = (u != 0.0 && isfinite(u)) ? t_exp * a / u : 0.0;
-  return isfinite(d) ? d : 0.0;
+  return m == G_COPY ? 1.0f : (m == G_NEG ? -1.0f : 0.0f);
 }
 
 // gh.mode[f], gh.src[f], gh.val[f] for a face index known only at run time,
@@ -63,8 +61,11 @@ struct CellIn {
 
 }  // namespace
 
+// UNIT: no density array and cp = 1 (alpha without its division by rho cp)
+template <bool UNIT>
 __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
-  __shared__ double sU[2][R3], sA[2][R3], sT[2][R3], sDa[2][R3];
+  __shared__ double sU[2][R3], sA[2][R3];
+  __shared__ float sT[2][R3], sDa[2][R3];
   __shared__ double sX[NFX], sY[NFY], sZ[R3];
   __shared__ double sRed[2][L3T / WAVE];
   const Geo& g = A.g;
@@ -82,6 +83,8 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
   const int gx = x0 - 1, gy = y0 - 1;
   const int harm = A.harmonic;
   const double dt = A.ic.dt;
+  const IterConst32 k32 = iter_const32(A.ic, A.dtau_scale, A.theta_scale);
+  const float t_exp = (float)A.t_exp;
 
   int rlx[2], rly[2];
   bool rv[2];
@@ -137,7 +140,8 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
     for (int s = 0; s < 2; ++s) {
       if (!rv[s]) continue;
       const int r = rly[s] * R3X + rlx[s];
-      double u = 0.0, a = 0.0, th = 0.0, da = 0.0;
+      double u = 0.0, a = 0.0;
+      float th = 0.0f, da = 0.0f;
       if (C[s].kind != 0) {
         if (C[s].kind == 1 || C[s].kind == -1) {
           const double us = C[s].v + (double)C[s].e;
@@ -145,12 +149,10 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
         } else {
           u = C[s].v;
         }
-        a = alpha_of<double>(A.kap, C[s].rho, u);
-        double dtau;
-        iter_params<double>(A.ic, C[s].rho, a, dtau, th);
-        dtau *= A.dtau_scale;
-        th *= A.theta_scale;
-        da = dadu3(A.t_exp, a, u);
+        a = alpha_of<double, UNIT>(A.kap, C[s].rho, u);
+        float Cc;
+        iter_params32(k32, (float)C[s].rho, (float)a, Cc, th);
+        da = dadu32(t_exp, (float)a, (float)u);
         const int i = gx + rlx[s], j = gy + rly[s];
         const bool gi = i < g.lo[0] || i > g.hi[0], gj = j < g.lo[1] || j > g.hi[1];
         const bool gk = p < g.lo[2] || p > g.hi[2];
@@ -160,15 +162,11 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
         if (!gi && !gj && !gk) {
           if (own_i && own_j && own_k) {
             A.u_new[c] = u;
-            A.a64[c] = a;
-            A.cc[c] = (float)(dtau / (1.0 + dtau / dt));
+            A.cc[c] = Cc;
           }
         } else if ((int)gi + (int)gj + (int)gk == 1) {
           // face ghosts: the block owning the adjacent column cells writes them
-          if ((gi && own_j && own_k) || (gj && own_i && own_k) || (gk && own_i && own_j)) {
-            A.u_new[c] = u;
-            A.a64[c] = a;
-          }
+          if ((gi && own_j && own_k) || (gj && own_i && own_k) || (gk && own_i && own_j)) A.u_new[c] = u;
         }
       }
       sU[buf][r] = u;
@@ -186,30 +184,24 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
   auto coefs = [&](int b0, int r0, int b1, int r1, int flo, int fhi, bool lo_face, bool hi_face, float* F0, float* F1,
                    float* F2, int64_t gc) {
     const double a0 = sA[b0][r0], a1 = sA[b1][r1];
-    const double abar = face_mean<double>(harm, a0, a1);
-    const double tbar = face_mean<double>(harm, sT[b0][r0], sT[b1][r1]);
-    double P = abar, M = abar;
+    float P = (float)face_mean<double>(harm, a0, a1), M = P;
     if (A.newton) {
-      const double sum = a0 + a1;
-      double d0 = 0.5, d1 = 0.5;
-      if (harm) {
-        d0 = (sum != 0.0) ? 2.0 * a1 * a1 / (sum * sum) : 0.0;
-        d1 = (sum != 0.0) ? 2.0 * a0 * a0 / (sum * sum) : 0.0;
-      }
-      const double du = sU[b1][r1] - sU[b0][r0];
-      P -= d0 * sDa[b0][r0] * du;
-      M += d1 * sDa[b1][r1] * du;
+      float d0, d1;
+      mean_weights32(harm, (float)a0, (float)a1, d0, d1);
+      const float du = (float)(sU[b1][r1] - sU[b0][r0]);
+      P = __builtin_fmaf(-d0 * sDa[b0][r0], du, P);
+      M = __builtin_fmaf(d1 * sDa[b1][r1], du, M);
     }
     if (lo_face) {
-      M -= gfac3(A.gh, flo) * P;
-      P = 0.0;
+      M = __builtin_fmaf(-gfac3(A.gh, flo), P, M);
+      P = 0.0f;
     } else if (hi_face) {
-      P -= gfac3(A.gh, fhi) * M;
-      M = 0.0;
+      P = __builtin_fmaf(-gfac3(A.gh, fhi), M, P);
+      M = 0.0f;
     }
-    F0[gc] = (float)P;
-    F1[gc] = (float)M;
-    F2[gc] = (float)(tbar / (1.0 + tbar));
+    F0[gc] = P;
+    F1[gc] = M;
+    F2[gc] = damping32(harm, sT[b0][r0], sT[b1][r1]);
   };
   auto faces = [&](int p, bool zf) {
     const int bc = p & 1, bm = (p + 1) & 1;
@@ -396,6 +388,13 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
     A.psum[b] = s;
     A.pmax[b] = m;
   }
+}
+
+void launch_dc_outer3d(const DcOuter3D& A, int nblk, bool unit, hipStream_t stream) {
+  if (unit)
+    hipLaunchKernelGGL(k_dc_outer3d<true>, dim3(nblk), dim3(L3T), 0, stream, A);
+  else
+    hipLaunchKernelGGL(k_dc_outer3d<false>, dim3(nblk), dim3(L3T), 0, stream, A);
 }
 
 }  // namespace cdiff

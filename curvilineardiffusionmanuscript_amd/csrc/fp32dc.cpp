@@ -416,15 +416,7 @@ bool cdiff_solver::lin_ok() const {
 
 int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, double* u_new, const double* uprev,
                                 int iter, const cdiff_opts& o, bool finalize) {
-  const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  if (!e && ndim == 3) {
-    // alpha over the full array once (corners and deeper halo layers keep it;
-    // the 3D pass refreshes the domain and the first ghost layer); the 2D pass
-    // keeps alpha in LDS only
-    hipLaunchKernelGGL(k_alpha_full, grid_n(nfull), dim3(256), 0, stream, g, kap, (const double*)rho,
-                       (const double*)nullptr, u_old, a64);
-    LAUNCH_CHECK("k_alpha_full");
-  }
+  // (the fused passes keep alpha in LDS; the step's epilogue forms the final one)
   const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
   if (ndim == 3) {
     const int nk = g.hi[2] - g.lo[2] + 1;
@@ -446,7 +438,6 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
     B.u_old = u_old;
     B.e = e;
     B.u_new = u_new;
-    B.a64 = a64;
     B.uprev = uprev;
     B.c = coef[0];
     B.cstride = g.total;
@@ -461,7 +452,7 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
     B.pmax = dc_pmax;
     const int nblk = (B.ntiles + 7) / 8 * 8;
     if (nblk > dc_nparts) return fail(CDIFF_HIP_ERROR, "partial-sum buffer too small");
-    hipLaunchKernelGGL(k_dc_outer3d, dim3(nblk), dim3(512), 0, stream, B);
+    launch_dc_outer3d(B, nblk, rho == nullptr && kap.cp == 1.0, stream);
     LAUNCH_CHECK("k_dc_outer3d");
     if (finalize) {
       hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, ctrl, (const double*)dc_psum,
