@@ -117,6 +117,7 @@ class Stats(C.Structure):
         ("outer", C.c_int32),
         ("warm_start", C.c_int32),
         ("fp64_fallback", C.c_int32),
+        ("warm_consulted", C.c_int32),
     ]
 
 

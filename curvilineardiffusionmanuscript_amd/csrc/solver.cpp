@@ -497,12 +497,15 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   // the prologue overwrites u_prev
   warm_pending = false;
   warm_r0_ready = false;
-  const bool warm_here = net && (o.precision == CDIFF_PREC_FP32_DC || o.precision == CDIFF_PREC_FP64) && ndim == 2 &&
-                         lin_ok();
-  // after predictions that lost to T^n or won by less than half, the network
-  // sits out 3, 7, 15 (at most 16) steps before it is consulted again
+  // (from the second step on: the network reads the previous increment
+  // T^n - T^{n-1} and was trained on steps that have one)
+  const bool warm_here = net && have_prev && (o.precision == CDIFF_PREC_FP32_DC || o.precision == CDIFF_PREC_FP64) &&
+                         ndim == 2 && lin_ok();
+  // after consultations that did not pay (warm_account), the network sits
+  // out 15, 31, ... (at most 255) steps before it is consulted again
   const bool warm_skipped = warm_here && warm_skip > 0;
   if (warm_skipped) --warm_skip;
+  st.warm_consulted = (warm_here && !warm_skipped) ? 1 : 0;
   if (warm_here && !warm_skipped) {
     if (!learn_ts) {
       HIP_TRY(hipMalloc((void**)&learn_ts, 2 * sizeof(double)));
@@ -696,6 +699,7 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   have_prev = true;  // u_prev = T^n, the previous level of the next step
   dt_prev = dt;
   t_written = true;
+  warm_account(st, dc);
   const auto t1 = std::chrono::steady_clock::now();
   st.time = std::chrono::duration<double>(t1 - t0).count();
   if (stats) *stats = st;

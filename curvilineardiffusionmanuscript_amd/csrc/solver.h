@@ -148,8 +148,12 @@ struct cdiff_solver {
   bool warm_pending = false;        // a prediction for this step is in the network's workspace
   bool dc_failed = false;  // the last mixed-precision step gave up (diverged or did not converge)
   bool warm_r0_ready = false;       // the feature pass left T^n's FP64 residual partial sums in place
-  int warm_rejects = 0;             // consecutive steps whose prediction lost to T^n
+  int warm_rejects = 0;             // consecutive consultations that did not pay
   int warm_skip = 0;                // steps left before the network is consulted again
+  double warm_gain = 0.0;           // decades of residual the prediction used at this step saved
+  double warm_r_pred = 0.0;         // residual of that predicted start
+  void warm_back_off();
+  void warm_account(const cdiff_stats& st, bool dc);
 
   std::string err;
 

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 
@@ -81,8 +82,50 @@ int cdiff_solver::learn_target(const double* ts_dev, float* tgt) {
 // After the first outer pass of an FP32_DC step (u[1] = T^n with its BC
 // ghosts, its residual norm in ctrl_h[2]): evaluate the residual of the
 // predicted start and keep whichever start has the smaller residual.
+namespace {
+
+// Work in units of one FP32 correction sweep at the same mesh size, from
+// measurements at 4096^2 on MI355X (all three scale with the cell count): an
+// FP64 APT iteration ~7.4 sweeps (~390 vs 53 us), a fused outer pass of the
+// defect correction ~7.4 sweeps, and one consultation of the network
+// (features, inference, the FP64 residual of the predicted start) ~55 sweeps
+// (~2.9 ms)
+constexpr double WORK_FP64_ITER = 7.4, WORK_OUTER_PASS = 7.4, WORK_CONSULT = 55.0;
+
+}  // namespace
+
+// after a consultation that did not pay, the network sits out 15, 31, 63,
+// ... (at most 255) steps before it is consulted again: where it never pays
+// (the mixed-precision hot disk) one trial costs ~55 sweeps, a few percent of
+// the 15 or more steps it then sits out
+void cdiff_solver::warm_back_off() {
+  ++warm_rejects;
+  warm_skip = std::min((16 << std::min(warm_rejects - 1, 4)) - 1, 255);
+}
+
+// A used prediction is judged by the work it saved: the step's own work per
+// decade of residual reduction times the decades the prediction took off the
+// start (log10 r(T^n) / r(prediction)), against the cost of the consultation.
+// The APT's iterations per decade are nearly fixed by the mesh and dt, so this
+// is the work a start from T^n would have added.  Deterministic: iteration
+// counts and residuals only.
+void cdiff_solver::warm_account(const cdiff_stats& st, bool dc) {
+  if (!st.warm_start) return;
+  const double work = dc ? st.niter + WORK_OUTER_PASS * st.outer : WORK_FP64_ITER * st.niter;
+  const double decades = std::max(1.0, std::log10(std::max(warm_r_pred, 1e-300) / std::max(st.abs_err, 1e-300)));
+  const double saved = work / decades * warm_gain;
+  if (tune.verbose)
+    std::fprintf(stderr, "[warm] saved ~%.1f sweep units (%.2f decades at %.1f per decade), consultation %.0f\n",
+                 saved, warm_gain, work / decades, WORK_CONSULT);
+  if (saved > WORK_CONSULT)
+    warm_rejects = 0;
+  else
+    warm_back_off();
+}
+
 int cdiff_solver::warm_start_apply(double dt, const cdiff_opts& o, bool* used) {
   *used = false;
+  warm_gain = 0.0;
   const double r0 = ctrl_h[2].abs_;
   launch_apply_prediction(g, u[1], net->delta, learn_ts, u[0], stream);
   LAUNCH_CHECK("k_apply_prediction");
@@ -92,26 +135,14 @@ int cdiff_solver::warm_start_apply(double dt, const cdiff_opts& o, bool* used) {
   HIP_TRY(hipStreamSynchronize(stream));
   if (tune.verbose)
     std::fprintf(stderr, "[dc] residual of the predicted start %.3e (of T^n %.3e)\n", ctrl_h[2].abs_, r0);
-  // the back-off: after a prediction that did not at least halve the
-  // residual, the network sits out 3, 7, 15 (at most 16) steps.  A
-  // consultation that loses costs ~3 ms at 4096^2 (inference, the residual
-  // of the predicted start and that of T^n again), about a third of a
-  // mixed-precision step, while a win saves a few percent of a step
-  auto back_off = [&]() {
-    ++warm_rejects;
-    warm_skip = std::min((2 << std::min(warm_rejects, 4)) - 1, 16);
-  };
   if (ctrl_h[2].abs_ < r0 && ctrl_h[2].status == CDIFF_OK) {
-    // the smaller residual wins; a marginal win does not pay for the next
-    // inference, so it counts as a loss for the back-off
+    // the smaller residual wins; whether it paid is judged at the step's end
     *used = true;
-    if (ctrl_h[2].abs_ <= 0.5 * r0)
-      warm_rejects = 0;
-    else
-      back_off();
+    warm_r_pred = ctrl_h[2].abs_;
+    warm_gain = std::log10(r0 / ctrl_h[2].abs_);
     return CDIFF_OK;
   }
-  back_off();
+  warm_back_off();
   // fallback: start from T^n (in u_prev) again
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl, o.rtol, o.atol);
   hipLaunchKernelGGL(k_copy, dim3((g.total + 255) / 256), dim3(256), 0, stream, g.total, (const double*)u_prev, u[0]);

@@ -486,7 +486,7 @@ def _opts(**kwargs) -> L.Opts:
 def _stats(st: L.Stats):
     return SimpleNamespace(
         rel_err=st.rel_err, abs_err=st.abs_err, niter=st.niter, time=st.time, nchecks=st.nchecks, outer=st.outer,
-        warm_start=bool(st.warm_start), fp64_fallback=bool(st.fp64_fallback),
+        warm_start=bool(st.warm_start), fp64_fallback=bool(st.fp64_fallback), warm_consulted=bool(st.warm_consulted),
     )
 
 

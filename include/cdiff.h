@@ -128,6 +128,8 @@ typedef struct {
   int32_t fp64_fallback; /* FP32_DC: 1 if the defect correction gave up (diverged or did not converge)
                           * and the step was redone by the FP64 APT from T^n; the other fields are
                           * then the FP64 APT's */
+  int32_t warm_consulted; /* 1 if the learned warm start was consulted at this step (its prediction
+                           * used or not); 0 when none is attached or it sat the step out */
 } cdiff_stats;
 
 /* Fields for cdiff_get_field / cdiff_set_field. */

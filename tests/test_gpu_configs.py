@@ -125,10 +125,17 @@ def test_config5_blobs_4096_with_trained_network(seed):
         err = float(np.max(np.abs(res[name][0] - ref)))
         print(f"|T_{name} - T_fp64| = {err:.3e} (bound {bound:.3e})")
         assert err <= bound
-    # the network's gain on the FP64 APT: predicted starts are used and save
-    # pseudo-iterations over the ten steps
+    # the network's gain on the FP64 APT: predicted starts are used and the
+    # pseudo-iterations they save over the ten steps exceed what every
+    # consultation cost (features, inference and the residual of the
+    # predicted start: ~2.9 ms at 4096^2, ~7.4 FP64 iterations)
     cold = sum(st.niter for st, _ in res["fp64"][2])
     warm = sum(st.niter for st, _ in res["fp64+learned"][2])
     used = sum(int(st.warm_start) for st, _ in res["fp64+learned"][2])
+    consulted = sum(int(st.warm_consulted) for st, _ in res["fp64+learned"][2])
+    t_cold = sum(st.time for st, _ in res["fp64"][2])
+    t_warm = sum(st.time for st, _ in res["fp64+learned"][2])
+    print(f"FP64 APT: {cold} iterations cold, {warm} with the network ({used} used of {consulted} consultations); "
+          f"{t_cold:.3f} s against {t_warm:.3f} s")
     assert used >= 3
-    assert warm <= 0.98 * cold
+    assert cold - warm > 7.4 * consulted

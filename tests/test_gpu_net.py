@@ -118,3 +118,38 @@ def test_warm_started_step_reaches_the_same_tolerance(precision):
     T_cold, T_warm = out[False][3], out[True][3]
     assert np.max(np.abs(T_warm - T_cold)) <= 6 * 2e-5 * np.sqrt(T0.size) * 1e-8 * 10
     net.close()
+
+
+def test_back_off_stops_charging_where_the_network_does_not_pay():
+    # the mixed-precision hot disk, where every prediction loses to T^n (or
+    # saves less work than its consultation costs): the network is first
+    # consulted at the second step (it reads the previous increment), then
+    # sits out 15 steps, then 31; the steps it sits out are the plain
+    # mixed-precision steps
+    net = LW.WarmStartNet.load()
+    mesh, T0 = hot_disk_2d(512, seed=0)
+    bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
+    law = S.PowerLawConductivity(1.0, 2.5, 0.0)
+    out = {}
+    for warm in (False, True):
+        sol = S.PseudoTransientSolver(mesh, bcs)
+        if warm:
+            net.attach(sol)
+        S.update_conductivity(sol, mesh, T0, 1.0, 1.0, law)
+        sol.set_temperature(T0)
+        dt, rec = 4e-6, []
+        for _ in range(20):
+            st, dt = sol.step_resident(dt, None, 1.0, law, rtol=0.0, atol=1e-8, max_iter=10**6, precision="fp32_dc")
+            rec.append((st.niter, st.outer, int(st.warm_start), int(st.warm_consulted), st.abs_err))
+            assert st.abs_err <= 1e-8
+        out[warm] = (rec, sol.get_temperature())
+        sol.close()
+    net.close()
+    consulted = [k for k, r in enumerate(out[True][0]) if r[3]]
+    used = [k for k, r in enumerate(out[True][0]) if r[2]]
+    print("consulted at steps", consulted, "used at", used)
+    assert consulted == [1, 17]
+    if not used:
+        assert [r[:2] for r in out[True][0]] == [r[:2] for r in out[False][0]]
+        np.testing.assert_array_equal(out[True][1], out[False][1])
+
