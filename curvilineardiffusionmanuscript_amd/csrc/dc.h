@@ -62,8 +62,9 @@ struct DcOuter2D {
 };
 // unit: no density array and cp = 1
 void launch_dc_outer2d(const DcOuter2D& A, int nblk, bool unit, hipStream_t stream);
-// Fused outer pass of the 3D step (dc_outer3d.hip): the same work over 64 x 8
-// columns marching up chunks of `ch` planes.
+// Fused outer pass of the 3D step (dc_outer3d.hip): the same work over 64 x
+// DC_OUTER3D_LY columns marching up chunks of `ch` planes.
+constexpr int DC_OUTER3D_LY = 8;
 struct DcOuter3D {
   Geo g;
   Ghost gh;

@@ -17,8 +17,9 @@
 //   residual of plane p-1 per column cell, in FP64: the reference residual
 //            (residuals.jl:24-47) with the 3D divergence of
 //            flux_divergence.jl:119-239, summed in the order of k_dc_residual;
-//            the faces of planes p-2 and p-1 it needs are carried in registers
-//            from the previous steps, so the face planes are single-buffered.
+//            the face planes are double-buffered in LDS (plane p's faces in
+//            buffer p & 1), so the residual reads the faces of planes p-1 and
+//            p there and carries only the face sums of plane p-2.
 // The cell inputs of the next plane step are loaded one step ahead, the
 // residual inputs at the start of the step that uses them.
 #include <limits>
@@ -31,10 +32,12 @@ namespace cdiff {
 
 namespace {
 
-constexpr int FXN = L3X + 1;  // xi faces per ring row: between region columns fx and fx+1
-constexpr int FYN = L3Y + 1;  // eta face rows: between region rows fy and fy+1
-constexpr int NFX = FXN * R3Y, NFY = R3X * FYN;
-constexpr int NFACE = NFX + NFY + R3;
+using OC = Col3<DC_OUTER3D_LY>;
+constexpr int OX = OC::X, OY = OC::Y, ORX = OC::RX, ORY = OC::RY, OR = OC::R, OT = OC::T;
+constexpr int FXN = OX + 1;  // xi faces per ring row: between region columns fx and fx+1
+constexpr int FYN = OY + 1;  // eta face rows: between region rows fy and fy+1
+constexpr int NFX = FXN * ORY, NFY = ORX * FYN;
+constexpr int NFACE = NFX + NFY + OR;
 
 __device__ __forceinline__ float gfac3(const Ghost& gh, int f) {
   const int m = gh.mode[f];
@@ -63,16 +66,16 @@ struct CellIn {
 
 // UNIT: no density array and cp = 1 (alpha without its division by rho cp)
 template <bool UNIT>
-__global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
-  __shared__ double sU[2][R3], sA[2][R3];
-  __shared__ float sT[2][R3], sDa[2][R3];
-  __shared__ double sX[NFX], sY[NFY], sZ[R3];
-  __shared__ double sRed[2][L3T / WAVE];
+__global__ __launch_bounds__(OT) void k_dc_outer3d(DcOuter3D A) {
+  __shared__ double sU[2][OR], sA[2][OR];
+  __shared__ float sT[2][OR], sDa[2][OR];
+  __shared__ double sX[2][NFX], sY[2][NFY], sZ[2][OR];
+  __shared__ double sRed[2][OT / WAVE];
   const Geo& g = A.g;
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
   Column3D col3;
-  if (!column_of(g, A.ntiles, A.chunks, A.stripsI, A.ch, col3)) {
+  if (!column_of<DC_OUTER3D_LY>(g, A.ntiles, A.chunks, A.stripsI, A.ch, col3)) {
     if (tid == 0) {
       A.psum[b] = 0.0;
       A.pmax[b] = 0.0;
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
   int rlx[2], rly[2];
   bool rv[2];
 #pragma unroll
-  for (int s = 0; s < 2; ++s) rv[s] = region_cell(s, tid, rlx[s], rly[s]);
+  for (int s = 0; s < 2; ++s) rv[s] = region_cell<DC_OUTER3D_LY>(s, tid, rlx[s], rly[s]);
 
   // ---- loads of the cells of plane p
   auto fetch_cells = [&](int p, CellIn* C) {
@@ -139,7 +142,7 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       if (!rv[s]) continue;
-      const int r = rly[s] * R3X + rlx[s];
+      const int r = rly[s] * ORX + rlx[s];
       double u = 0.0, a = 0.0;
       float th = 0.0f, da = 0.0f;
       if (C[s].kind != 0) {
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
         const int i = gx + rlx[s], j = gy + rly[s];
         const bool gi = i < g.lo[0] || i > g.hi[0], gj = j < g.lo[1] || j > g.hi[1];
         const bool gk = p < g.lo[2] || p > g.hi[2];
-        const bool own_i = rlx[s] >= 1 && rlx[s] <= L3X, own_j = rly[s] >= 1 && rly[s] <= L3Y;
+        const bool own_i = rlx[s] >= 1 && rlx[s] <= OX, own_j = rly[s] >= 1 && rly[s] <= OY;
         const bool own_k = p >= z0 && p <= z1;
         const int64_t c = at(g, i, j, p);
         if (!gi && !gj && !gk) {
@@ -208,50 +211,50 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
     const bool pdom = p >= g.lo[2] && p <= g.hi[2];
     const bool own_p = p >= z0 && p <= z1;
 #pragma unroll 1
-    for (int q = 0; q < (NFACE + L3T - 1) / L3T; ++q) {
-      const int t = tid + q * L3T;
+    for (int q = 0; q < (NFACE + OT - 1) / OT; ++q) {
+      const int t = tid + q * OT;
       if (t < NFX) {
         const int fx = t % FXN, cy = t / FXN;
         const int i = gx + fx, j = gy + cy;
         double v = 0.0;
         if (pdom && i >= g.lo[0] - 1 && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1]) {
-          const int r = cy * R3X + fx;
+          const int r = cy * ORX + fx;
           v = -face_mean<double>(harm, sA[bc][r], sA[bc][r + 1]) * (sU[bc][r + 1] - sU[bc][r]);
-          if (own_p && cy >= 1 && cy <= L3Y && (fx >= 1 || i == g.lo[0] - 1))
+          if (own_p && cy >= 1 && cy <= OY && (fx >= 1 || i == g.lo[0] - 1))
             coefs(bc, r, bc, r + 1, 0, 1, i == g.lo[0] - 1, i == g.hi[0], A.f + 0 * A.fstride, A.f + 1 * A.fstride, A.f + 2 * A.fstride, at(g, i, j, p));
         }
-        sX[t] = v;
+        sX[bc][t] = v;
       } else if (t < NFX + NFY) {
         const int u = t - NFX;
-        const int fx = u % R3X, fy = u / R3X;
+        const int fx = u % ORX, fy = u / ORX;
         const int i = gx + fx, j = gy + fy;
         double v = 0.0;
         if (pdom && i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] - 1 && j <= g.hi[1]) {
-          const int r = fy * R3X + fx;
-          v = -face_mean<double>(harm, sA[bc][r], sA[bc][r + R3X]) * (sU[bc][r + R3X] - sU[bc][r]);
-          if (own_p && fx >= 1 && fx <= L3X && (fy >= 1 || j == g.lo[1] - 1))
-            coefs(bc, r, bc, r + R3X, 2, 3, j == g.lo[1] - 1, j == g.hi[1], A.f + 3 * A.fstride, A.f + 4 * A.fstride, A.f + 5 * A.fstride, at(g, i, j, p));
+          const int r = fy * ORX + fx;
+          v = -face_mean<double>(harm, sA[bc][r], sA[bc][r + ORX]) * (sU[bc][r + ORX] - sU[bc][r]);
+          if (own_p && fx >= 1 && fx <= OX && (fy >= 1 || j == g.lo[1] - 1))
+            coefs(bc, r, bc, r + ORX, 2, 3, j == g.lo[1] - 1, j == g.hi[1], A.f + 3 * A.fstride, A.f + 4 * A.fstride, A.f + 5 * A.fstride, at(g, i, j, p));
         }
-        sY[u] = v;
+        sY[bc][u] = v;
       } else if (t < NFACE) {
         const int r = t - NFX - NFY;
-        const int lx = r % R3X, ly = r / R3X;
+        const int lx = r % ORX, ly = r / ORX;
         const int i = gx + lx, j = gy + ly, k = p - 1;  // face between planes k and k+1, stored at k
         double v = 0.0;
         if (zf && i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1] && k >= g.lo[2] - 1 &&
             k <= g.hi[2]) {
           v = -face_mean<double>(harm, sA[bm][r], sA[bc][r]) * (sU[bc][r] - sU[bm][r]);
           const bool own_k = (k >= z0 && k <= z1) || k == g.lo[2] - 1;
-          if (own_k && lx >= 1 && lx <= L3X && ly >= 1 && ly <= L3Y)
+          if (own_k && lx >= 1 && lx <= OX && ly >= 1 && ly <= OY)
             coefs(bm, r, bc, r, 4, 5, k == g.lo[2] - 1, k == g.hi[2], A.f + 6 * A.fstride, A.f + 7 * A.fstride, A.f + 8 * A.fstride, at(g, i, j, k));
         }
-        sZ[r] = v;
+        sZ[bc][r] = v;
       }
     }
   };
 
   // ---- per column cell: residual inputs of plane p
-  const int tx = tid % L3X, ty = tid / L3X;
+  const int tx = tid % OX, ty = tid / OX;
   const int ci = x0 + tx, cj = y0 + ty;
   const bool col = ci <= g.hi[0] && cj <= g.hi[1];
   struct ResIn {
@@ -266,47 +269,60 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
     R.src = A.S ? A.S[c] : 0.0;
   };
 
-  // faces around the column cell, carried between plane steps
+  // faces around the column cell (LDS, buffer of the plane)
   struct Plane {
     double xc, xw, xeta, yc, ys, yxi;  // xi / eta faces of the plane
-    double xsum, ysum;                 // X(0) + X(-1), Y(0) + Y(0,-1)
   };
   struct ZSt {
     double c, e, w, n, s;  // zeta faces stored at a plane: (0,0), (+1,0), (-1,0), (0,+1), (0,-1)
   };
   const int x_r = (ty + 1) * FXN + tx + 1;  // X(0,0) of the column cell
-  const int y_r = (ty + 1) * R3X + tx + 1;  // Y(0,0) and Z(0,0)
-  auto read_plane = [&](Plane& P) {
-    P.xc = sX[x_r];
-    P.xw = sX[x_r - 1];
-    P.xeta = (sX[x_r + FXN] + sX[x_r + FXN - 1]) - (sX[x_r - FXN] + sX[x_r - FXN - 1]);
-    P.yc = sY[y_r];
-    P.ys = sY[y_r - R3X];
-    P.yxi = (sY[y_r + 1] + sY[y_r - R3X + 1]) - (sY[y_r - 1] + sY[y_r - R3X - 1]);
-    P.xsum = P.xc + P.xw;
-    P.ysum = P.yc + P.ys;
+  const int y_r = (ty + 1) * ORX + tx + 1;  // Y(0,0) and Z(0,0)
+  auto read_plane = [&](int fb) {
+    const double* X = sX[fb];
+    const double* Y = sY[fb];
+    Plane P;
+    P.xc = X[x_r];
+    P.xw = X[x_r - 1];
+    P.xeta = (X[x_r + FXN] + X[x_r + FXN - 1]) - (X[x_r - FXN] + X[x_r - FXN - 1]);
+    P.yc = Y[y_r];
+    P.ys = Y[y_r - ORX];
+    P.yxi = (Y[y_r + 1] + Y[y_r - ORX + 1]) - (Y[y_r - 1] + Y[y_r - ORX - 1]);
+    return P;
   };
-  auto read_z = [&](ZSt& Z) {
-    Z.c = sZ[y_r];
-    Z.e = sZ[y_r + 1];
-    Z.w = sZ[y_r - 1];
-    Z.n = sZ[y_r + R3X];
-    Z.s = sZ[y_r - R3X];
+  // X(0) + X(-1) and Y(0) + Y(0,-1) of a plane
+  auto xsum_of = [&](int fb) { return sX[fb][x_r] + sX[fb][x_r - 1]; };
+  auto ysum_of = [&](int fb) { return sY[fb][y_r] + sY[fb][y_r - ORX]; };
+  auto read_z = [&](int fb) {
+    const double* Z = sZ[fb];
+    ZSt q;
+    q.c = Z[y_r];
+    q.e = Z[y_r + 1];
+    q.w = Z[y_r - 1];
+    q.n = Z[y_r + ORX];
+    q.s = Z[y_r - ORX];
+    return q;
   };
 
   double sq = 0.0, mx = 0.0;
-  // residual of plane k from the faces of planes k-1 (sums xm, ym), k (pc),
-  // k+1 (pn) and the zeta faces stored at k-1 (zm) and k (zc)
-  auto residual = [&](int k, const ResIn& R, double xm, double ym, const Plane& pc, const Plane& pn, const ZSt& zm,
-                      const ZSt& zc, double un, double an) {
+  // residual of plane k (faces in buffer k & 1, those of plane k+1 and the
+  // zeta faces stored at k in buffer (k+1) & 1, the zeta faces stored at k-1
+  // in buffer k & 1) with the face sums xm, ym of plane k-1
+  auto residual = [&](int k, double xm, double ym, double un, double an) {
     if (!col) return;
+    ResIn R{};
+    fetch_res(k, R);
+    const int kb = k & 1, nb = (k + 1) & 1;
+    const Plane pc = read_plane(kb);
+    const double pn_xsum = xsum_of(nb), pn_ysum = ysum_of(nb);
+    const ZSt zm = read_z(kb), zc = read_z(nb);
     const double a1 = R.c[0] * (pc.xc - pc.xw);
     const double a2 = R.c[1] * (pc.yc - pc.ys);
     const double a3 = R.c[2] * (zc.c - zm.c);
     const double qi_eta = R.c[3] * pc.xeta;
-    const double qi_zeta = R.c[4] * (pn.xsum - xm);
+    const double qi_zeta = R.c[4] * (pn_xsum - xm);
     const double qj_xi = R.c[3] * pc.yxi;
-    const double qj_zeta = R.c[5] * (pn.ysum - ym);
+    const double qj_zeta = R.c[5] * (pn_ysum - ym);
     const double qk_eta = R.c[5] * ((zc.n + zm.n) - (zc.s + zm.s));
     const double qk_xi = R.c[4] * ((zc.e + zm.e) - (zc.w + zm.w));
     const double m1 = R.c[6] * (0.5 * (pc.xc + pc.xw));
@@ -330,8 +346,6 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
 
   // ---- prologue: plane step z0-1 (no zeta faces), then z0
   CellIn C[2];
-  Plane pc{}, pn{};
-  ZSt zm{}, zc{};
   double xm, ym;                // face sums of the plane below the next residual's
   double u_c = 0.0, a_c = 0.0;  // column cell of the plane whose residual is next
   fetch_cells(z0 - 1, C);
@@ -340,9 +354,8 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
   __syncthreads();
   faces(z0 - 1, false);
   __syncthreads();
-  read_plane(pn);
-  xm = pn.xsum;
-  ym = pn.ysum;
+  xm = xsum_of((z0 - 1) & 1);
+  ym = ysum_of((z0 - 1) & 1);
   cells(z0, C);
   u_c = u_own;
   a_c = a_own;
@@ -350,24 +363,16 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
   __syncthreads();
   faces(z0, true);
   __syncthreads();
-  read_plane(pc);
-  read_z(zm);
   // ---- plane steps z0+1 .. z1+1: residual of plane p-1
   for (int p = z0 + 1; p <= z1 + 1; ++p) {
-    ResIn R{};
-    fetch_res(p - 1, R);  // in flight during the cells and faces of plane p
     cells(p, C);
     if (p <= z1) fetch_cells(p + 1, C);
     __syncthreads();
     faces(p, true);
     __syncthreads();
-    read_plane(pn);
-    read_z(zc);
-    residual(p - 1, R, xm, ym, pc, pn, zm, zc, u_c, a_c);
-    xm = pc.xsum;
-    ym = pc.ysum;
-    pc = pn;
-    zm = zc;
+    residual(p - 1, xm, ym, u_c, a_c);
+    xm = xsum_of((p - 1) & 1);
+    ym = ysum_of((p - 1) & 1);
     u_c = u_own;
     a_c = a_own;
   }
@@ -381,7 +386,7 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
   __syncthreads();
   if (tid == 0) {
     double s = 0.0, m = 0.0;
-    for (int w = 0; w < L3T / WAVE; ++w) {
+    for (int w = 0; w < OT / WAVE; ++w) {
       s += sRed[0][w];
       m = nanmax(m, sRed[1][w]);
     }
@@ -392,9 +397,9 @@ __global__ __launch_bounds__(L3T) void k_dc_outer3d(DcOuter3D A) {
 
 void launch_dc_outer3d(const DcOuter3D& A, int nblk, bool unit, hipStream_t stream) {
   if (unit)
-    hipLaunchKernelGGL(k_dc_outer3d<true>, dim3(nblk), dim3(L3T), 0, stream, A);
+    hipLaunchKernelGGL(k_dc_outer3d<true>, dim3(nblk), dim3(OT), 0, stream, A);
   else
-    hipLaunchKernelGGL(k_dc_outer3d<false>, dim3(nblk), dim3(L3T), 0, stream, A);
+    hipLaunchKernelGGL(k_dc_outer3d<false>, dim3(nblk), dim3(OT), 0, stream, A);
 }
 
 }  // namespace cdiff

@@ -92,8 +92,10 @@ int cdiff_solver::ensure_fp32() {
   const int64_t nfull = (int64_t)g.n[0] * g.n[1] * g.n[2];
   // one partial per 256 cells (per-cell residual kernels) or per tile of the
   // fused outer pass, whose block count is rounded up to a multiple of 8
-  // (64 x 8 tiles; in 3D at most one plane per chunk)
-  const int64_t tiles = (int64_t)((g.hi[0] - g.lo[0] + 64) / 64) * ((g.hi[1] - g.lo[1] + 8) / 8) *
+  // (64 x DC_OUTER2D_TY tiles in 2D, 64 x DC_OUTER3D_LY columns of at least
+  // one plane in 3D)
+  const int tile_rows = ndim == 3 ? std::min(DC_OUTER2D_TY, DC_OUTER3D_LY) : DC_OUTER2D_TY;
+  const int64_t tiles = (int64_t)((g.hi[0] - g.lo[0] + 64) / 64) * ((g.hi[1] - g.lo[1] + tile_rows) / tile_rows) *
                         (ndim == 3 ? (g.hi[2] - g.lo[2] + 1) : 1);
   const int nres = (int)std::max<int64_t>((nfull + 255) / 256, (tiles + 7) / 8 * 8);
   if (nres > dc_nparts) {
@@ -432,7 +434,7 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
     B.ch = std::max(1, std::min(nk, (int)tune.dc_outer_planes3d));
     B.stripsI = (ni + 63) / 64;
     B.chunks = (nk + B.ch - 1) / B.ch;
-    B.ntiles = B.stripsI * ((nj + 7) / 8) * B.chunks;
+    B.ntiles = B.stripsI * ((nj + DC_OUTER3D_LY - 1) / DC_OUTER3D_LY) * B.chunks;
     B.t_exp = kap.t_exp;
     B.rho = rho;
     B.u_old = u_old;
