@@ -153,7 +153,7 @@ def run_gpu(args, mesh, T0, method, net=None):
 # one launch runs and the cdiff_bench_sweeps mode bits that launch it
 KERNELS = {
     ("fp64", 2): ("sweep2d_pair_kernel<double, false, 8>", 2, 0 | 2),
-    ("fp32_dc", 2): ("sweep2d_lin_march_kernel<6, false>", 6, 4),
+    ("fp32_dc", 2): ("sweep2d_lin_march_kernel<6, false, 1>", 6, 4),
     ("fp64", 3): ("sweep3d_col_kernel<double, false, true>", 1, 0),
     ("fp32_dc", 3): ("sweep3d_lin2_kernel<false>", 2, 4),
 }

@@ -100,7 +100,15 @@ __device__ __forceinline__ bool c1_cell(int slot, int tid, int& lx, int& ly) {
 
 template <typename T, bool CHECK, int TY>
 __global__ __launch_bounds__(PTX* TY) void sweep2d_pair_kernel(Sweep2DArgs<T> A) {
-  const IterConst ic = A.ic_dev ? *A.ic_dev : A.ic;
+  // the iteration constants: kernel arguments, or device memory in a
+  // captured graph (read field by field: a select of the two blocks would
+  // hold them in vector registers and cost a wavefront per SIMD)
+  IterConst ic = A.ic;
+  if (A.ic_dev) {
+    ic.L = __builtin_nontemporal_load(&A.ic_dev->L);
+    ic.Vpdtau = __builtin_nontemporal_load(&A.ic_dev->Vpdtau);
+    ic.dt = __builtin_nontemporal_load(&A.ic_dev->dt);
+  }
   constexpr int LY = TY + 4;
   constexpr int NC2 = LX * LY;
   constexpr int NTHR = PTX * TY;

@@ -72,7 +72,15 @@ __device__ __forceinline__ bool approx_eps(T x, T y) {
 
 template <typename T, bool CHECK, int TY>
 __global__ __launch_bounds__(TX* TY) void sweep2d_tile_kernel(Sweep2DArgs<T> A) {
-  const IterConst ic = A.ic_dev ? *A.ic_dev : A.ic;
+  // the iteration constants: kernel arguments, or device memory in a
+  // captured graph (read field by field: a select of the two blocks would
+  // hold them in vector registers and cost a wavefront per SIMD)
+  IterConst ic = A.ic;
+  if (A.ic_dev) {
+    ic.L = __builtin_nontemporal_load(&A.ic_dev->L);
+    ic.Vpdtau = __builtin_nontemporal_load(&A.ic_dev->Vpdtau);
+    ic.dt = __builtin_nontemporal_load(&A.ic_dev->dt);
+  }
   constexpr int CY = TY + 2;             // ringed rows
   constexpr int NCELL = CX * CY;         // cells with ring
   constexpr int NFX = FXN * CY;          // xi faces
