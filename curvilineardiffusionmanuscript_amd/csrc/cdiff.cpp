@@ -205,7 +205,7 @@ void cdiff_default_tuning(cdiff_tuning* t) {
   t->dc_cfl_scale = 1.2;
   t->dc_dtau_scale = 1.3;
   t->dc_theta_scale = 1.0;
-  t->graphs = 1;
+  t->graphs = 0;
 }
 
 int cdiff_set_tuning(cdiff_solver* h, const cdiff_tuning* t) {

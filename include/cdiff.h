@@ -182,7 +182,8 @@ typedef struct {
   int32_t verbose;          /* 1: per-round diagnostics of the defect correction on stderr */
   int32_t exact_pow;        /* 1: pow() for T^(n+1/2) instead of T^n sqrt(T) (within 2 ulp) */
   int32_t graphs;           /* 1: 2D FP64 blocks of 16 pseudo-iterations without a check replayed
-                               from captured HIP graphs (default), 0: launched one by one */
+                               from captured HIP graphs; 0: launched one by one (default: the
+                               same speed or faster where measured, 128^2 to 4096^2) */
   double dc_eta;            /* inner solves stop at eta x the outer residual; 0 = auto (1e-3 2D, 1e-2 3D) */
   double dc_floor;          /* ... or at floor x atol: 0.3 */
   double dc_cfl_scale;      /* pseudo-time CFL of the correction sweeps over the step's: 1.2 */
