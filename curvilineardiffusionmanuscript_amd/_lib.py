@@ -103,7 +103,7 @@ class Tuning(C.Structure):
         "sweep2d", "tile_rows", "march_rows64", "pairs", "sweep3d", "planes3d", "seg_min", "seg_max",
         "dc_coefficient_form", "dc_newton", "dc_fused3d", "dc_depth2d", "dc_layout2d", "dc_march_rows",
         "dc_launch_iters", "dc_depth3d", "dc_planes3d", "dc_planes3d_pair", "dc_outer_planes3d", "dc_check",
-        "dc_max_outer", "verbose", "exact_pow", "graphs")] + [(n, C.c_double) for n in (
+        "dc_max_outer", "verbose", "exact_pow", "graphs", "warm_policy")] + [(n, C.c_double) for n in (
         "dc_eta", "dc_floor", "dc_cfl_scale", "dc_dtau_scale", "dc_theta_scale")]
 
 

@@ -188,7 +188,7 @@ void cdiff_default_tuning(cdiff_tuning* t) {
   t->dc_coefficient_form = 1;
   t->dc_newton = 1;
   t->dc_fused3d = 1;
-  t->dc_depth2d = 6;
+  t->dc_depth2d = 8;
   t->dc_layout2d = 2;
   t->dc_march_rows = 0;
   t->dc_launch_iters = 0;
@@ -206,6 +206,7 @@ void cdiff_default_tuning(cdiff_tuning* t) {
   t->dc_dtau_scale = 1.3;
   t->dc_theta_scale = 1.0;
   t->graphs = 0;
+  t->warm_policy = 0;
 }
 
 int cdiff_set_tuning(cdiff_solver* h, const cdiff_tuning* t) {
@@ -224,6 +225,7 @@ int cdiff_set_tuning(cdiff_solver* h, const cdiff_tuning* t) {
   if (v.dc_layout2d < 0 || v.dc_layout2d > 2) return bad("dc_layout2d must be 0, 1 or 2");
   if (v.dc_depth3d < 1 || v.dc_depth3d > 2) return bad("dc_depth3d must be 1 or 2");
   if (v.dc_launch_iters < 0 || v.dc_check < 0 || v.dc_max_outer < 1) return bad("dc counts out of range");
+  if (v.warm_policy < 0 || v.warm_policy > 1) return bad("warm_policy must be 0 or 1");
   if (!(v.dc_eta >= 0.0 && v.dc_eta < 1.0) || !(v.dc_floor >= 0.0) || !(v.dc_cfl_scale > 0.0) ||
       !(v.dc_dtau_scale > 0.0) || !(v.dc_theta_scale > 0.0))
     return bad("dc_eta in [0, 1), dc_floor >= 0 and positive scales required");

@@ -121,11 +121,10 @@ int cdiff_solver::ensure_fp32() {
   // coefficient-form sweep: no periodic faces
   lin = lin_ok();
   if (lin && ndim == 2) {
-    // six iterations per launch, marching up strips of 52 columns (4096^2:
-    // 57-58 us per iteration; 80.8 at depth 4, 71.0 at depth 8, whose 183
-    // VGPRs leave two wavefronts per SIMD).  On wave-aligned 52 x 20 tiles
-    // the same launch takes 79 us per iteration (92 at depth 4, 83 at
-    // depth 8; 100 at depth 4 on 64 x 16 tiles): tiles re-read their ring
+    // eight iterations per launch, marching up strips of 48 columns (4096^2:
+    // 42.4 us per iteration; 53.1 at depth 6, 75.5 at depth 4).  On
+    // wave-aligned 52 x 20 tiles six iterations take 79 us per iteration (100
+    // at depth 4 on 64 x 16 tiles): tiles re-read their ring
     lin_k = tune.dc_depth2d;
     const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
     lin_layout = tune.dc_layout2d;
@@ -137,10 +136,14 @@ int cdiff_solver::ensure_fp32() {
     lin_tiles = ((ni + cols - 1) / cols) * lin_chunks;
     if (lin_layout == 2) {
       // marching sweep: strips of 64 - 2 lin_k columns; chunks of rows sized
-      // for about 12 wavefronts per CU, at least 2 lin_k rows each
-      // (a multiple of 8 chunks when there are that many: one set per XCD)
+      // for about 24 wavefronts per CU at depth 8 (two resident per SIMD:
+      // the grid runs in about three rounds; 4096^2: 64 rows per chunk, 42.4
+      // us per iteration against 46.3 at 48 rows, 48.6 at 80, 54.6 at 96) and
+      // 12 at depth 6 (three resident per SIMD; 128 rows, 53.1 us against
+      // 55.6 at 64), at least 2 lin_k rows each (a multiple of 8 chunks when
+      // there are that many: one set per XCD)
       march_strips = (ni + 63 - 2 * lin_k) / (64 - 2 * lin_k);
-      int want = std::max(1, 12 * ncu / march_strips);
+      int want = std::max(1, (lin_k >= 8 ? 24 : 12) * ncu / march_strips);
       if (want >= 8) want -= want % 8;
       march_rows = std::max(2 * lin_k, (nj + want - 1) / want);
       if (tune.dc_march_rows > 0) march_rows = std::max(2 * lin_k, (int)tune.dc_march_rows);
@@ -547,13 +550,14 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   // inner stop at eta times the outer residual, checked every ichk sweeps; in
   // 3D (one sweep per launch) shorter inner solves checked more often need
   // fewer sweeps in total (256^3 spots: 48 -> 32 sweeps per step); in 2D
-  // every second six-iteration launch checks (a launch that ends early on a
-  // check costs nearly a full one)
+  // a check every 8 to 12 sweeps: every eight-iteration launch checks
+  // (4096^2 hot disk: 110 sweeps and 7.64 ms per step, against 125 sweeps and
+  // 8.03 ms checking every second launch), every second six-iteration one
   const double eta = tune.dc_eta > 0.0 ? tune.dc_eta : (ndim == 3 ? 1e-2 : 1e-3);
   // the last correction only needs to reach the outer tolerance: inner
   // sweeps stop at floor_frac * atol at the latest
   const double floor_frac = tune.dc_floor;
-  const int ichk = tune.dc_check > 0 ? (int)tune.dc_check : (ndim == 3 ? 2 : 2 * lin_k);
+  const int ichk = tune.dc_check > 0 ? (int)tune.dc_check : (ndim == 3 ? 2 : lin_k * std::max(1, 12 / lin_k));
   const int max_outer = tune.dc_max_outer;
   const int fixed_inner = o.inner_iters;
   const int64_t total_cap = (int64_t)o.max_iter;
@@ -632,6 +636,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   // or has not converged after max_outer rounds
   dc_failed = false;
   const double r_first = ctrl_h[2].abs_;
+  step_r_start = r_first;
   auto failed_now = [&]() {
     const Ctrl& c = ctrl_h[2];
     return c.status == CDIFF_NONFINITE || !(c.abs_ == c.abs_) || c.abs_ > 100.0 * r_first;

@@ -335,6 +335,7 @@ int cdiff_set_warm_start(cdiff_solver* h, cdiff_net* n) {
   h->net = n;
   h->warm_rejects = 0;
   h->warm_skip = 0;
+  h->warm_decades = 1.0;
   return CDIFF_OK;
 }
 

@@ -502,11 +502,16 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   const bool warm_here = net && have_prev && (o.precision == CDIFF_PREC_FP32_DC || o.precision == CDIFF_PREC_FP64) &&
                          ndim == 2 && lin_ok();
   // after consultations that did not pay (warm_account), the network sits
-  // out 15, 31, ... (at most 255) steps before it is consulted again
-  const bool warm_skipped = warm_here && warm_skip > 0;
+  // out 15, 31, ... (at most 255) steps before it is consulted again; and it
+  // is consulted only where the saving expected from it (the last step's work
+  // per decade of residual reduction times the decades a prediction takes
+  // off) exceeds what a consultation costs
+  const bool always = tune.warm_policy == 1;
+  const bool warm_skipped = warm_here && !always && warm_skip > 0;
   if (warm_skipped) --warm_skip;
-  st.warm_consulted = (warm_here && !warm_skipped) ? 1 : 0;
-  if (warm_here && !warm_skipped) {
+  const bool consult = warm_here && !warm_skipped && (always || warm_pays());
+  st.warm_consulted = consult ? 1 : 0;
+  if (consult) {
     if (!learn_ts) {
       HIP_TRY(hipMalloc((void**)&learn_ts, 2 * sizeof(double)));
       owned.push_back(learn_ts);
@@ -627,6 +632,7 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
   const bool dc = o.precision == CDIFF_PREC_FP32_DC && !fallback;
   st.fp64_fallback = fallback ? 1 : 0;
   if (!dc) {
+    step_r_start = c.init;
     st.rel_err = c.rel;
     st.abs_err = c.abs_;
     st.niter = c.final_iter;

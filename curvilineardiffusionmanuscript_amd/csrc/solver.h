@@ -152,8 +152,13 @@ struct cdiff_solver {
   int warm_skip = 0;                // steps left before the network is consulted again
   double warm_gain = 0.0;           // decades of residual the prediction used at this step saved
   double warm_r_pred = 0.0;         // residual of that predicted start
+  double warm_decades = 1.0;        // decades a consultation is expected to take off the start
+  double work_per_decade = 0.0;     // the last step's work (correction-sweep units) per decade of
+                                    // residual reduction; 0 before the first step
+  double step_r_start = 0.0;        // FP64 residual at the start of the current step
   void warm_back_off();
   void warm_account(const cdiff_stats& st, bool dc);
+  bool warm_pays() const;  // expected saving of a consultation above its cost
 
   std::string err;
 

@@ -107,20 +107,43 @@ void cdiff_solver::warm_back_off() {
 // decade of residual reduction times the decades the prediction took off the
 // start (log10 r(T^n) / r(prediction)), against the cost of the consultation.
 // The APT's iterations per decade are nearly fixed by the mesh and dt, so this
-// is the work a start from T^n would have added.  Deterministic: iteration
-// counts and residuals only.
+// is the work a start from T^n would have added.  Every step also records its
+// work per decade, from which the next step decides whether a consultation
+// can pay at all (warm_pays).  Deterministic: iteration counts and residuals
+// only.
 void cdiff_solver::warm_account(const cdiff_stats& st, bool dc) {
-  if (!st.warm_start) return;
   const double work = dc ? st.niter + WORK_OUTER_PASS * st.outer : WORK_FP64_ITER * st.niter;
-  const double decades = std::max(1.0, std::log10(std::max(warm_r_pred, 1e-300) / std::max(st.abs_err, 1e-300)));
-  const double saved = work / decades * warm_gain;
+  const double r_end = std::max(st.abs_err, 1e-300);
+  const double r_start = st.warm_start ? warm_r_pred : step_r_start;
+  const double decades = std::log10(std::max(r_start, 1e-300) / r_end);
+  // (a step that starts converged, or nearly so, says nothing about the rate)
+  if (std::isfinite(decades) && decades >= 0.5 && st.niter > 0) work_per_decade = work / decades;
+  if (!st.warm_start) return;
+  const double saved = work_per_decade * warm_gain;
+  warm_decades = warm_gain;
   if (tune.verbose)
     std::fprintf(stderr, "[warm] saved ~%.1f sweep units (%.2f decades at %.1f per decade), consultation %.0f\n",
-                 saved, warm_gain, work / decades, WORK_CONSULT);
+                 saved, warm_gain, work_per_decade, WORK_CONSULT);
   if (saved > WORK_CONSULT)
     warm_rejects = 0;
   else
     warm_back_off();
+}
+
+// Whether a consultation is expected to pay: the last step's work per decade
+// times the decades a prediction is expected to take off (1 before the first
+// used prediction; then what the last one took off) against its cost.  On the
+// FP64 APT a decade costs ~100-200 sweep units at 4096^2 and a consultation
+// pays; the mixed-precision step needs ~15-30 units per decade, so a
+// prediction would have to take 2-4 decades off the start, more than the
+// BF16 network's 0.7-1.4.
+bool cdiff_solver::warm_pays() const {
+  const bool pays = work_per_decade * warm_decades > WORK_CONSULT;
+  if (tune.verbose)
+    std::fprintf(stderr, "[warm] expected saving %.1f sweep units (%.1f per decade x %.2f decades), consultation %.0f: %s\n",
+                 work_per_decade * warm_decades, work_per_decade, warm_decades, WORK_CONSULT,
+                 pays ? "consult" : "skip");
+  return pays;
 }
 
 int cdiff_solver::warm_start_apply(double dt, const cdiff_opts& o, bool* used) {

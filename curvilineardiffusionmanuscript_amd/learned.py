@@ -284,21 +284,25 @@ def evaluate(net, problem: str, n: int, steps: int, dt0: float, seed: int = 0, a
         law = S.PowerLawConductivity(1.0, float(rng.uniform(1.0, 3.0)), 0.0)
         bcs = [S.NeumannBC()] * 4
     out = {}
+    # "+learned": consulted where a saving above the consultation's cost is
+    # expected (the default); "+learned(always)": at every step from the second
     for name, prec, warm in (("fp64", "fp64", False), ("fp64+learned", "fp64", True), ("fp32_dc", "fp32_dc", False),
-                             ("fp32_dc+learned", "fp32_dc", True)):
-        sol = S.PseudoTransientSolver(mesh, bcs)
+                             ("fp32_dc+learned", "fp32_dc", True), ("fp32_dc+learned(always)", "fp32_dc", True)):
+        sol = S.PseudoTransientSolver(mesh, bcs, tuning={"warm_policy": 1} if name.endswith("(always)") else None)
         if warm:
             net.attach(sol)
         S.update_conductivity(sol, mesh, T0, 1.0, 1.0, law)
         sol.set_temperature(T0)
-        dt, its, used, tt = dt0, [], [], 0.0
+        dt, its, used, cons, tt = dt0, [], [], [], 0.0
         for _ in range(steps):
             t0 = time.perf_counter()
             st, dt = sol.step_resident(dt, None, 1.0, law, rtol=0.0, atol=atol, max_iter=10**7, precision=prec)
             tt += time.perf_counter() - t0
             its.append(st.niter)
             used.append(int(st.warm_start))
-        out[name] = dict(iters=its, warm_used=used, s_per_step=tt / steps, T=sol.get_temperature())
+            cons.append(int(st.warm_consulted))
+        out[name] = dict(iters=its, warm_used=used, warm_consulted=cons, s_per_step=tt / steps,
+                         T=sol.get_temperature())
         sol.close()
     ref = out["fp64"]["T"]
     scale = float(np.max(np.abs(ref)))

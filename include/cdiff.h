@@ -169,7 +169,7 @@ typedef struct {
   int32_t dc_coefficient_form; /* 1: coefficient-form sweeps where no face is periodic (default), 0: conductivity form */
   int32_t dc_newton;        /* 1: d(alpha)/du terms in the correction operator (default) */
   int32_t dc_fused3d;       /* 1: fused 3D outer pass (default), 0: separate kernels */
-  int32_t dc_depth2d;       /* iterations per 2D correction launch: 2, 4, 6 (default) or 8 */
+  int32_t dc_depth2d;       /* iterations per 2D correction launch: 2, 4, 6 or 8 (default) */
   int32_t dc_layout2d;      /* 0: 64-column tiles, 1: wave-aligned tiles, 2: marching strips (default) */
   int32_t dc_march_rows;    /* rows per marching chunk, 0 = auto */
   int32_t dc_launch_iters;  /* iterations per launch, 1 .. depth; 0 = the depth */
@@ -177,13 +177,17 @@ typedef struct {
   int32_t dc_planes3d;      /* planes per chunk of the one-iteration 3D sweep, 0 = auto */
   int32_t dc_planes3d_pair; /* planes per chunk of the two-iteration 3D sweep, 0 = auto */
   int32_t dc_outer_planes3d; /* planes per chunk of the fused 3D outer pass: 32 */
-  int32_t dc_check;         /* correction sweeps between checks, 0 = auto (2 x depth in 2D, 2 in 3D) */
+  int32_t dc_check;         /* correction sweeps between checks, 0 = auto (2D: the depth times max(1, 12 / depth); 3D: 2) */
   int32_t dc_max_outer;     /* correction rounds before the step is redone by the FP64 APT: 100 */
   int32_t verbose;          /* 1: per-round diagnostics of the defect correction on stderr */
   int32_t exact_pow;        /* 1: pow() for T^(n+1/2) instead of T^n sqrt(T) (within 2 ulp) */
   int32_t graphs;           /* 1: 2D FP64 blocks of 16 pseudo-iterations without a check replayed
                                from captured HIP graphs; 0: launched one by one (default: the
                                same speed or faster where measured, 128^2 to 4096^2) */
+  int32_t warm_policy;      /* learned warm start: 0 consults the network where the saving expected
+                               from it exceeds a consultation's cost, and sits out after
+                               consultations that did not pay (default); 1 consults it at every
+                               step it applies to (evaluation, tests) */
   double dc_eta;            /* inner solves stop at eta x the outer residual; 0 = auto (1e-3 2D, 1e-2 3D) */
   double dc_floor;          /* ... or at floor x atol: 0.3 */
   double dc_cfl_scale;      /* pseudo-time CFL of the correction sweeps over the step's: 1.2 */

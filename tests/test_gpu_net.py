@@ -98,7 +98,9 @@ def test_warm_started_step_reaches_the_same_tolerance(precision):
     law = S.PowerLawConductivity(1.0, 2.5, 0.0)
     out = {}
     for warm in (False, True):
-        sol = S.PseudoTransientSolver(mesh, bcs)
+        # consulted at every step (warm_policy 1): the cost rule would keep a
+        # network this small out of steps this cheap
+        sol = S.PseudoTransientSolver(mesh, bcs, tuning={"warm_policy": 1} if warm else None)
         if warm:
             net.attach(sol)
         S.update_conductivity(sol, mesh, T0, 1.0, 1.0, law)
@@ -113,6 +115,7 @@ def test_warm_started_step_reaches_the_same_tolerance(precision):
         sol.close()
     print("cold", out[False][:2], "warm", out[True][:2])
     assert not any(out[False][1])
+    assert any(out[True][1])
     assert all(e <= 1e-8 for e in out[True][2])
     # both trajectories satisfy the same FP64 residual tolerance every step
     T_cold, T_warm = out[False][3], out[True][3]
@@ -120,36 +123,37 @@ def test_warm_started_step_reaches_the_same_tolerance(precision):
     net.close()
 
 
-def test_back_off_stops_charging_where_the_network_does_not_pay():
-    # the mixed-precision hot disk, where every prediction loses to T^n (or
-    # saves less work than its consultation costs): the network is first
-    # consulted at the second step (it reads the previous increment), then
-    # sits out 15 steps, then 31; the steps it sits out are the plain
-    # mixed-precision steps
+def test_network_sits_out_where_it_cannot_pay():
+    # the mixed-precision hot disk: a decade of residual reduction costs the
+    # step ~10-30 correction sweeps, so a prediction would have to take 2-5
+    # decades off the start to repay the ~55 sweeps a consultation costs;
+    # the network is never consulted and every step is the plain
+    # mixed-precision step, bit for bit.  Consulted anyway (warm_policy 1),
+    # the steps still end at the tolerance.
     net = LW.WarmStartNet.load()
     mesh, T0 = hot_disk_2d(512, seed=0)
     bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
     law = S.PowerLawConductivity(1.0, 2.5, 0.0)
     out = {}
-    for warm in (False, True):
-        sol = S.PseudoTransientSolver(mesh, bcs)
+    for name, warm, tuning in (("plain", False, None), ("auto", True, None), ("always", True, {"warm_policy": 1})):
+        sol = S.PseudoTransientSolver(mesh, bcs, tuning=tuning)
         if warm:
             net.attach(sol)
         S.update_conductivity(sol, mesh, T0, 1.0, 1.0, law)
         sol.set_temperature(T0)
         dt, rec = 4e-6, []
-        for _ in range(20):
+        for _ in range(12):
             st, dt = sol.step_resident(dt, None, 1.0, law, rtol=0.0, atol=1e-8, max_iter=10**6, precision="fp32_dc")
             rec.append((st.niter, st.outer, int(st.warm_start), int(st.warm_consulted), st.abs_err))
             assert st.abs_err <= 1e-8
-        out[warm] = (rec, sol.get_temperature())
+        out[name] = (rec, sol.get_temperature())
         sol.close()
     net.close()
-    consulted = [k for k, r in enumerate(out[True][0]) if r[3]]
-    used = [k for k, r in enumerate(out[True][0]) if r[2]]
-    print("consulted at steps", consulted, "used at", used)
-    assert consulted == [1, 17]
-    if not used:
-        assert [r[:2] for r in out[True][0]] == [r[:2] for r in out[False][0]]
-        np.testing.assert_array_equal(out[True][1], out[False][1])
-
+    for name in ("auto", "always"):
+        print(name, "consulted at steps", [k for k, r in enumerate(out[name][0]) if r[3]],
+              "used at", [k for k, r in enumerate(out[name][0]) if r[2]])
+    assert not any(r[3] for r in out["auto"][0])
+    assert [r[:2] for r in out["auto"][0]] == [r[:2] for r in out["plain"][0]]
+    np.testing.assert_array_equal(out["auto"][1], out["plain"][1])
+    # from the second step on (the network reads the previous increment)
+    assert [r[3] for r in out["always"][0]] == [0] + [1] * 11

@@ -159,8 +159,9 @@ def test_tuning_round_trip_and_range_checks():
     mesh, _ = gaussian_2d(32)
     dev = S.PseudoTransientSolver(mesh, [S.NeumannBC()] * 4, tuning={"dc_eta": 2e-3, "verbose": 0})
     t = dev.get_tuning()
-    assert t["dc_eta"] == 2e-3 and t["dc_depth2d"] == 6 and t["tile_rows"] == 8 and t["dc_layout2d"] == 2
-    for bad in ({"tile_rows": 5}, {"dc_depth2d": 3}, {"dc_eta": 1.0}, {"seg_min": 0}, {"dc_cfl_scale": 0.0}):
+    assert t["dc_eta"] == 2e-3 and t["dc_depth2d"] == 8 and t["tile_rows"] == 8 and t["dc_layout2d"] == 2
+    for bad in ({"tile_rows": 5}, {"dc_depth2d": 3}, {"dc_eta": 1.0}, {"seg_min": 0}, {"dc_cfl_scale": 0.0},
+                {"warm_policy": 2}):
         with pytest.raises(S.SolverAssertionError, match="cdiff_set_tuning"):
             dev.set_tuning(**bad)
         assert dev.get_tuning() == t  # unchanged
