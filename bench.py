@@ -150,10 +150,12 @@ def run_gpu(args, mesh, T0, method, net=None):
 
 
 # dominant kernel of each method and dimension, with the pseudo-iterations
-# one launch runs and the cdiff_bench_sweeps mode bits that launch it
+# one launch runs and the cdiff_bench_sweeps mode bits that launch it (bit 0:
+# the residual-check variant -- every launch of the 2D correction solve
+# checks; bit 1: two FP64 iterations per launch; bit 2: the correction sweep)
 KERNELS = {
     ("fp64", 2): ("sweep2d_pair_kernel<double, false, 8>", 2, 0 | 2),
-    ("fp32_dc", 2): ("sweep2d_lin_march_kernel<8, false, 1>", 8, 4),
+    ("fp32_dc", 2): ("sweep2d_lin_march_kernel<8, true, 1>", 8, 4 | 1),
     ("fp64", 3): ("sweep3d_col_kernel<double, false, true>", 1, 0),
     ("fp32_dc", 3): ("sweep3d_lin2_kernel<false>", 2, 4),
 }
