@@ -66,6 +66,7 @@ EXPORTS = (
     "cdiff_net_last_error",
     "cdiff_set_warm_start",
     "cdiff_net_set_gate",
+    "cdiff_net_set_loss",
 )
 
 
@@ -190,6 +191,7 @@ def load(path: str = LIB_PATH):
     L.cdiff_net_last_error.restype = C.c_char_p
     L.cdiff_set_warm_start.argtypes = [_vp, _vp]
     L.cdiff_net_set_gate.argtypes = [_vp, C.c_float]
+    L.cdiff_net_set_loss.argtypes = [_vp, C.c_int]
     for name in EXPORTS:
         if name not in ("cdiff_default_opts", "cdiff_default_tuning", "cdiff_last_error", "cdiff_status_string",
                         "cdiff_net_last_error"):

@@ -1,6 +1,6 @@
 // learn.hip -- kernels of the learned warm start (see learn.h).
 //
-// The three 32-channel layers run on MFMA (v_mfma_f32_16x16x32_bf16, BF16
+// The seven convolutions run on MFMA (v_mfma_f32_16x16x32_bf16, BF16
 // operands, FP32 accumulation) as implicit GEMMs:
 //   forward / backward-data   D[co][px] = sum_k W[co][k] * X[px + off(k)][ci(k)]
 //                             k = (tap, ci): one 32-wide K-block per tap (a
@@ -11,9 +11,10 @@
 //                             bias gradient is one more column with X = 1
 // MFMA operand layout (16x16x32): lane l holds A[l%16][8(l/16) + j],
 // B[8(l/16) + j][l%16] (j = 0..7) and receives D[4(l/16) + r][l%16].
-// The 1-channel head (288 MACs per pixel) runs on MFMA for inference (one
-// M-tile, row 0 real) and on the vector ALUs in training, where it also forms
-// the loss gradient.
+// The 4-output head (1152 MACs per level-1 pixel) runs on MFMA for inference
+// (one M-tile, rows 0-3 real) and on the vector ALUs in training, where it
+// also forms the loss gradient.  Pooling, up-sampling, the skip additions and
+// their gradients are elementwise passes over 8-channel (16-byte) groups.
 #include <cmath>
 
 #include "learn.h"
@@ -34,31 +35,36 @@ __device__ __forceinline__ bf16x8 load8(const __bf16* p) {
   return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
 }
 
+__device__ __forceinline__ void store8(__bf16* p, bf16x8 v) {
+  *reinterpret_cast<uint4*>(p) = __builtin_bit_cast(uint4, v);
+}
+
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
 
 __device__ __forceinline__ float bf(const __bf16 v) { return (float)v; }
 
-constexpr int CONV_RB = 16;  // output rows per forward block
+constexpr int CONV_RB = 16;     // output rows per forward block
+constexpr int CG = NET_C / 8;   // 8-channel groups per pixel
 
-// 3x3 convolution, CIN -> 32 channels, as an implicit GEMM on MFMA.  A block
+// 3x3 convolution, 32 -> 32 channels, as an implicit GEMM on MFMA.  A block
 // of 4 waves owns a 64-pixel-wide strip of CONV_RB rows; wave w computes the
 // 16-pixel N-tile w of every row for both 16-channel M-tiles, with the
 // weights (A operands) in registers for the whole strip.  The input rows pass
-// through a 3-row LDS ring (66 pixels x CIN channels): each input row is read
+// through a 3-row LDS ring (66 pixels x 32 channels): each input row is read
 // from memory once per strip, the next row's load is in flight while the
 // current row is computed, and the nine taps read their B operands (8
 // channels of one pixel per lane) from LDS.  The MFMA chain per output tile
 // is the same as a direct evaluation (taps in order), so results do not
 // depend on the staging.
-//   MODE 0: y = relu(acc + bias)      MODE 1: y = acc where mask > 0, else 0
-template <int CIN, int MODE>
+//   MODE 0: y = relu(acc + bias)   MODE 1: y = acc where mask > 0, else 0
+//   MODE 2: y = acc
+template <int MODE>
 __global__ __launch_bounds__(256) void k_conv3x3(Img m, const __bf16* __restrict__ X, const __bf16* __restrict__ Wp,
                                                  const float* __restrict__ bias, const __bf16* __restrict__ mask,
                                                  __bf16* __restrict__ Y) {
-  constexpr int KB = CIN == 32 ? 9 : 3;  // K-blocks of 32
+  constexpr int KB = 9;                  // K-blocks of 32: one per tap
   constexpr int RX = 66;                 // ring row: pixels x0-1 .. x0+64
-  constexpr int CH = CIN / 8;            // 16-byte chunks per pixel
-  constexpr int NCH = RX * CH;           // chunks per ring row
+  constexpr int NCH = RX * CG;           // 16-byte chunks per ring row
   constexpr int NLD = (NCH + 255) / 256;
   __shared__ bf16x8 sRow[3][NCH];
   const int tid = threadIdx.x;
@@ -80,9 +86,9 @@ __global__ __launch_bounds__(256) void k_conv3x3(Img m, const __bf16* __restrict
       const int t = tid + q * 256;
       pre[q] = zero8();
       if (t < NCH) {
-        const int px = t / CH, c = t % CH;
+        const int px = t / CG, c = t % CG;
         const int xi = x0 - 1 + px;
-        if (yi <= m.H && xi <= m.W) pre[q] = load8(X + img_pix(m, yi, xi) * CIN + 8 * c);
+        if (yi <= m.H && xi <= m.W) pre[q] = load8(X + img_pix(m, yi, xi) * NET_C + 8 * c);
       }
     }
   };
@@ -107,10 +113,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(Img m, const __bf16* __restrict
     f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
-      const int tap = CIN == 32 ? kb : 4 * kb + lg;
-      const int c0 = CIN == 32 ? lg : 0;
-      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-      const bf16x8 bv = tap < 9 ? sRow[(y + dy + 3) % 3][(xo + 1 + dx) * CH + c0] : zero8();
+      const int dy = kb / 3 - 1, dx = kb % 3 - 1;
+      const bf16x8 bv = sRow[(y + dy + 3) % 3][(xo + 1 + dx) * CG + lg];
       acc[0] = mfma16(a[0][kb], bv, acc[0]);
       acc[1] = mfma16(a[1][kb], bv, acc[1]);
     }
@@ -126,7 +130,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(Img m, const __bf16* __restrict
           float v = acc[mt][r];
           if (MODE == 0)
             v = fmaxf(v + bias[co + r], 0.f);
-          else
+          else if (MODE == 1)
             v = bf(mask[p * NET_C + co + r]) > 0.f ? v : 0.f;
           o[r] = (__bf16)v;
         }
@@ -137,81 +141,279 @@ __global__ __launch_bounds__(256) void k_conv3x3(Img m, const __bf16* __restrict
   }
 }
 
-// A pixel is active when the explicit-Euler increment (feature 5, scaled by
-// its domain maximum) exceeds `gate` in its 3x3 neighbourhood; elsewhere the
-// prediction is 0 (no change), so the network adds no background error.
-__device__ __forceinline__ bool active(const Img& m, const __bf16* __restrict__ X0, int y, int x, float gate) {
-  float e = 0.f;
-  for (int tap = 0; tap < 9; ++tap)
-    e = fmaxf(e, fabsf(bf(X0[img_pix(m, y + tap / 3 - 1, x + tap % 3 - 1) * NET_C0 + 5])));
-  return e > gate;
+// ---- pooling, up-sampling, skips and their gradients (one thread per
+//      8-channel group of a pixel)
+
+// P[y][x] = ((X[2y][2x] + X[2y][2x+1]) + (X[2y+1][2x] + X[2y+1][2x+1])) / 4,
+// pixels beyond an odd edge read the zero border
+__global__ void k_pool(Img m, Img mc, const __bf16* __restrict__ X, __bf16* __restrict__ P) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)mc.H * mc.W * CG) return;
+  const int c = (int)(e % CG);
+  const int64_t px = e / CG;
+  const int y = (int)(px / mc.W), x = (int)(px % mc.W);
+  const bf16x8 a = load8(X + img_pix(m, 2 * y, 2 * x) * NET_C + 8 * c);
+  const bf16x8 b = load8(X + img_pix(m, 2 * y, 2 * x + 1) * NET_C + 8 * c);
+  const bf16x8 d = load8(X + img_pix(m, 2 * y + 1, 2 * x) * NET_C + 8 * c);
+  const bf16x8 f = load8(X + img_pix(m, 2 * y + 1, 2 * x + 1) * NET_C + 8 * c);
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) o[j] = (__bf16)(((bf(a[j]) + bf(b[j])) + (bf(d[j]) + bf(f[j]))) * 0.25f);
+  store8(P + img_pix(mc, y, x) * NET_C + 8 * c, o);
 }
 
-// head forward (+ loss gradient when a target is given)
-__global__ __launch_bounds__(256) void k_head_fwd(Img m, const __bf16* __restrict__ X3, const __bf16* __restrict__ w4,
-                                                  const float* __restrict__ params, const __bf16* __restrict__ X0,
-                                                  float gate, float* __restrict__ delta,
-                                                  const float* __restrict__ target, float* __restrict__ G4,
-                                                  double* __restrict__ loss_part) {
-  __shared__ float sw[9 * NET_C];
-  __shared__ double sred[256 / WAVE];
-  for (int t = threadIdx.x; t < 9 * NET_C; t += 256) sw[t] = bf(w4[t]);
+// A[y][x] = C[y/2][x/2] + S[y][x]
+__global__ void k_up_add(Img m, Img mc, const __bf16* __restrict__ C, const __bf16* __restrict__ S,
+                         __bf16* __restrict__ A) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)m.H * m.W * CG) return;
+  const int c = (int)(e % CG);
+  const int64_t px = e / CG;
+  const int y = (int)(px / m.W), x = (int)(px % m.W);
+  const int64_t p = img_pix(m, y, x) * NET_C + 8 * c;
+  const bf16x8 u = load8(C + img_pix(mc, y >> 1, x >> 1) * NET_C + 8 * c);
+  const bf16x8 s = load8(S + p);
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) o[j] = (__bf16)(bf(u[j]) + bf(s[j]));
+  store8(A + p, o);
+}
+
+// GE[y][x] = (GA[y][x] + GP[y/2][x/2] / 4) * (E[y][x] > 0)
+__global__ void k_grad_skip(Img m, Img mc, const __bf16* __restrict__ GA, const __bf16* __restrict__ GP,
+                            const __bf16* __restrict__ E, __bf16* __restrict__ GE) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)m.H * m.W * CG) return;
+  const int c = (int)(e % CG);
+  const int64_t px = e / CG;
+  const int y = (int)(px / m.W), x = (int)(px % m.W);
+  const int64_t p = img_pix(m, y, x) * NET_C + 8 * c;
+  const bf16x8 ga = load8(GA + p), em = load8(E + p);
+  const bf16x8 gp = load8(GP + img_pix(mc, y >> 1, x >> 1) * NET_C + 8 * c);
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) o[j] = (__bf16)(bf(em[j]) > 0.f ? bf(ga[j]) + 0.25f * bf(gp[j]) : 0.f);
+  store8(GE + p, o);
+}
+
+// GU[y][x] = ((GA[2y][2x] + GA[2y][2x+1]) + (GA[2y+1][2x] + GA[2y+1][2x+1])) * (U[y][x] > 0)
+__global__ void k_grad_up(Img m, Img mc, const __bf16* __restrict__ GA, const __bf16* __restrict__ U,
+                          __bf16* __restrict__ GU) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)mc.H * mc.W * CG) return;
+  const int c = (int)(e % CG);
+  const int64_t px = e / CG;
+  const int y = (int)(px / mc.W), x = (int)(px % mc.W);
+  const bf16x8 a = load8(GA + img_pix(m, 2 * y, 2 * x) * NET_C + 8 * c);
+  const bf16x8 b = load8(GA + img_pix(m, 2 * y, 2 * x + 1) * NET_C + 8 * c);
+  const bf16x8 d = load8(GA + img_pix(m, 2 * y + 1, 2 * x) * NET_C + 8 * c);
+  const bf16x8 f = load8(GA + img_pix(m, 2 * y + 1, 2 * x + 1) * NET_C + 8 * c);
+  const int64_t q = img_pix(mc, y, x) * NET_C + 8 * c;
+  const bf16x8 um = load8(U + q);
+  bf16x8 o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) o[j] = (__bf16)(bf(um[j]) > 0.f ? (bf(a[j]) + bf(b[j])) + (bf(d[j]) + bf(f[j])) : 0.f);
+  store8(GU + q, o);
+}
+
+// level-0 features (8 channels) into the level-1 block layout; feature 7 is
+// the largest |feature 5| over the cell's 3x3 neighbourhood (its activity)
+__global__ void k_s2d(Img m0, Img m1, const __bf16* __restrict__ X0, __bf16* __restrict__ XS) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)m0.H * m0.W) return;
+  const int y = (int)(e / m0.W), x = (int)(e % m0.W);
+  const int s = 2 * (y & 1) + (x & 1);
+  bf16x8 v = load8(X0 + img_pix(m0, y, x) * NET_C0);
+  float act = 0.f;
+  for (int tap = 0; tap < 9; ++tap)
+    act = fmaxf(act, fabsf(bf(X0[img_pix(m0, y + tap / 3 - 1, x + tap % 3 - 1) * NET_C0 + 5])));
+  v[7] = (__bf16)act;
+  store8(XS + img_pix(m1, y >> 1, x >> 1) * NET_C + 8 * s, v);
+}
+
+// ---- head
+
+// A cell is active when the explicit-Euler increment (feature 5, scaled by
+// its domain maximum) exceeds `gate` in its 3x3 neighbourhood: feature 7 of
+// the cell, read from the level-1 block layout; elsewhere the prediction is 0
+// (no change), so the network adds no background error.
+__device__ __forceinline__ bool active(const Img& m1, const __bf16* __restrict__ XS, int y, int x, float gate) {
+  return bf(XS[img_pix(m1, y >> 1, x >> 1) * NET_C + 8 * (2 * (y & 1) + (x & 1)) + 7]) > gate;
+}
+
+// head forward for training (vector ALU): one thread per level-1 pixel, its
+// four outputs for the cells of its 2x2 block
+__global__ __launch_bounds__(256) void k_head_fwd(Img m0, Img m1, const __bf16* __restrict__ U1,
+                                                  const __bf16* __restrict__ wh, const float* __restrict__ params,
+                                                  const __bf16* __restrict__ XS, float gate,
+                                                  float* __restrict__ delta) {
+  __shared__ float sw[NET_HO * 9 * NET_C];
+  for (int t = threadIdx.x; t < NET_HO * 9 * NET_C; t += 256) sw[t] = bf(wh[t]);
   __syncthreads();
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t npx = (int64_t)m.H * m.W;
-  double l = 0.0;
-  if (idx < npx) {
-    const int y = (int)(idx / m.W), x = (int)(idx % m.W);
-    float acc = params[NP_B4];
-    for (int tap = 0; tap < 9; ++tap) {
-      const __bf16* src = X3 + img_pix(m, y + tap / 3 - 1, x + tap % 3 - 1) * NET_C;
+  if (idx >= (int64_t)m1.H * m1.W) return;
+  const int y = (int)(idx / m1.W), x = (int)(idx % m1.W);
+  float acc[NET_HO];
 #pragma unroll
-      for (int q = 0; q < NET_C / 8; ++q) {
-        const bf16x8 v = load8(src + 8 * q);
+  for (int s = 0; s < NET_HO; ++s) acc[s] = params[NP_BH + s];
+  for (int tap = 0; tap < 9; ++tap) {
+    const __bf16* src = U1 + img_pix(m1, y + tap / 3 - 1, x + tap % 3 - 1) * NET_C;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc += bf(v[j]) * sw[tap * NET_C + 8 * q + j];
-      }
-    }
-    const bool on = active(m, X0, y, x, gate);
-    if (!on) acc = 0.f;
-    delta[idx] = acc;
-    if (target) {
-      const float d = acc - target[idx];
-      G4[img_pix(m, y, x)] = on ? 2.f * d / (float)npx : 0.f;
-      l = (double)d * (double)d;
+    for (int q = 0; q < CG; ++q) {
+      const bf16x8 v = load8(src + 8 * q);
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int s = 0; s < NET_HO; ++s) acc[s] += bf(v[j]) * sw[(s * 9 + tap) * NET_C + 8 * q + j];
     }
   }
-  if (!target) return;
-  l = wave_sum(l);
-  if ((threadIdx.x & (WAVE - 1)) == 0) sred[threadIdx.x / WAVE] = l;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < 256 / WAVE; ++i) s += sred[i];
-    loss_part[blockIdx.x] = s;
+#pragma unroll
+  for (int s = 0; s < NET_HO; ++s) {
+    const int fy = 2 * y + (s >> 1), fx = 2 * x + (s & 1);
+    if (fy < m0.H && fx < m0.W) delta[(int64_t)fy * m0.W + fx] = active(m1, XS, fy, fx, gate) ? acc[s] : 0.f;
   }
 }
 
-// Head forward for inference on MFMA: the 1-channel 3x3 convolution as the
-// implicit GEMM of k_conv3x3 with one 16-row M-tile whose row 0 holds the head
-// weights (rows 1-15 zero), the same strip / LDS-ring structure, and the bias
-// and activity gate applied to row 0.  Lanes 0-15 of each wave hold row 0 of
-// their 16 pixels.  (Training keeps k_head_fwd, which also forms the loss
-// gradient; the two differ only in the FP32 summation order.)
-__global__ __launch_bounds__(256) void k_head_mfma(Img m, const __bf16* __restrict__ X3, const __bf16* __restrict__ w4,
-                                                   const float* __restrict__ params, const __bf16* __restrict__ X0,
-                                                   float gate, float* __restrict__ delta) {
-  constexpr int KB = 9, RX = 66, CH = NET_C / 8, NCH = RX * CH, NLD = (NCH + 255) / 256;
+// ---- the training loss: |A e|^2 / |A target|^2, e = delta - target, with
+//      (A v)(c) = v(c) + sum_f D_f (v(c) - v(nbr_f)) over the faces of c that
+//      have a neighbour in the image, D_f = expm1(feature 3) on the east
+//      face, expm1(feature 4) on the north face (the face diffusion numbers)
+
+__device__ __forceinline__ float feat(const Img& m1, const __bf16* __restrict__ XS, int y, int x, int f) {
+  return bf(XS[img_pix(m1, y >> 1, x >> 1) * NET_C + 8 * (2 * (y & 1) + (x & 1)) + f]);
+}
+
+// (A v)(y, x) for v given cell-wise by `val`
+template <class V>
+__device__ __forceinline__ float apply_A(const Img& m0, const Img& m1, const __bf16* __restrict__ XS, int y, int x,
+                                         V val) {
+  const float vc = val(y, x);
+  float r = vc;
+  if (x + 1 < m0.W) r += expm1f(feat(m1, XS, y, x, 3)) * (vc - val(y, x + 1));
+  if (x > 0) r += expm1f(feat(m1, XS, y, x - 1, 3)) * (vc - val(y, x - 1));
+  if (y + 1 < m0.H) r += expm1f(feat(m1, XS, y, x, 4)) * (vc - val(y + 1, x));
+  if (y > 0) r += expm1f(feat(m1, XS, y - 1, x, 4)) * (vc - val(y - 1, x));
+  return r;
+}
+
+// re = A e per cell, and per-block partial sums of (A e)^2 and (A target)^2;
+// with RESIDUAL false (the mean squared error of the increment) A is the
+// identity and (A target)^2 counts 1 per cell
+template <bool RESIDUAL>
+__global__ __launch_bounds__(256) void k_res_fwd(Img m0, Img m1, const __bf16* __restrict__ XS,
+                                                 const float* __restrict__ delta, const float* __restrict__ target,
+                                                 float* __restrict__ re, double* __restrict__ part) {
+  __shared__ double sred[2][256 / WAVE];
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double se = 0.0, st = 0.0;
+  if (idx < (int64_t)m0.H * m0.W) {
+    const int y = (int)(idx / m0.W), x = (int)(idx % m0.W);
+    float ae, at;
+    if (RESIDUAL) {
+      ae = apply_A(m0, m1, XS, y, x, [&](int yy, int xx) {
+        const int64_t q = (int64_t)yy * m0.W + xx;
+        return delta[q] - target[q];
+      });
+      at = apply_A(m0, m1, XS, y, x, [&](int yy, int xx) { return target[(int64_t)yy * m0.W + xx]; });
+    } else {
+      ae = delta[idx] - target[idx];
+      at = 1.f;
+    }
+    re[idx] = ae;
+    se = (double)ae * ae;
+    st = (double)at * at;
+  }
+  se = wave_sum(se);
+  st = wave_sum(st);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    sred[0][threadIdx.x / WAVE] = se;
+    sred[1][threadIdx.x / WAVE] = st;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < 256 / WAVE; ++i) {
+      a += sred[0][i];
+      b += sred[1][i];
+    }
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// sums[0] = sum (A e)^2, sums[1] = sum (A target)^2 (fixed order); the loss
+__global__ void k_res_sums(const double* __restrict__ part, int n, double* __restrict__ sums,
+                           double* __restrict__ loss) {
+  __shared__ double s[2][256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + o];
+      s[1][threadIdx.x] += s[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sums[0] = s[0][0];
+    sums[1] = s[1][0];
+    *loss = s[1][0] > 0.0 ? s[0][0] / s[1][0] : 0.0;
+  }
+}
+
+// G4 (level-1 bordered, 4 channels) = 2 (A re) / sums[1] on active cells, 0
+// elsewhere (and on the cells beyond an odd edge)
+template <bool RESIDUAL>
+__global__ __launch_bounds__(256) void k_res_bwd(Img m0, Img m1, const __bf16* __restrict__ XS,
+                                                 const float* __restrict__ re, const double* __restrict__ sums,
+                                                 float gate, float* __restrict__ G4) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (int64_t)m1.H * m1.W) return;
+  const int y = (int)(idx / m1.W), x = (int)(idx % m1.W);
+  const double b = sums[1];
+  const float scale = b > 0.0 ? (float)(2.0 / b) : 0.f;
+  float g[NET_HO];
+#pragma unroll
+  for (int s = 0; s < NET_HO; ++s) {
+    const int fy = 2 * y + (s >> 1), fx = 2 * x + (s & 1);
+    g[s] = 0.f;
+    if (fy < m0.H && fx < m0.W && active(m1, XS, fy, fx, gate))
+      g[s] = scale * (RESIDUAL ? apply_A(m0, m1, XS, fy, fx,
+                                         [&](int yy, int xx) { return re[(int64_t)yy * m0.W + xx]; })
+                               : re[(int64_t)fy * m0.W + fx]);
+  }
+  *reinterpret_cast<float4*>(G4 + img_pix(m1, y, x) * NET_HO) = make_float4(g[0], g[1], g[2], g[3]);
+}
+
+// Head forward for inference on MFMA: the 4-output 3x3 convolution as the
+// implicit GEMM of k_conv3x3 with one 16-row M-tile whose rows 0-3 hold the
+// head weights (rows 4-15 zero), the same strip / LDS-ring structure, and the
+// bias and activity gate applied to rows 0-3.  Lanes 0-15 of each wave hold
+// rows 0-3 of their 16 pixels.  (Training keeps k_head_fwd, which also forms
+// the loss gradient; the two differ only in the FP32 summation order.)
+__global__ __launch_bounds__(256) void k_head_mfma(Img m0, Img m1, const __bf16* __restrict__ U1,
+                                                   const __bf16* __restrict__ wh, const float* __restrict__ params,
+                                                   const __bf16* __restrict__ XS, float gate,
+                                                   float* __restrict__ delta) {
+  constexpr int KB = 9, RX = 66, NCH = RX * CG, NLD = (NCH + 255) / 256;
   __shared__ bf16x8 sRow[3][NCH];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
   const int x0 = blockIdx.x * 64, y0 = blockIdx.y * CONV_RB;
-  const int y1 = min(y0 + CONV_RB, m.H);
+  const int y1 = min(y0 + CONV_RB, m1.H);
   bf16x8 a[KB];
 #pragma unroll
-  for (int kb = 0; kb < KB; ++kb) a[kb] = lr == 0 ? load8(w4 + kb * NET_C + 8 * lg) : zero8();
-  const float b4 = params[NP_B4];
+  for (int kb = 0; kb < KB; ++kb) a[kb] = lr < NET_HO ? load8(wh + (lr * 9 + kb) * NET_C + 8 * lg) : zero8();
+  float bh[NET_HO];
+#pragma unroll
+  for (int s = 0; s < NET_HO; ++s) bh[s] = params[NP_BH + s];
 
   bf16x8 pre[NLD];
   auto fetch = [&](int yi) {
@@ -220,9 +422,9 @@ __global__ __launch_bounds__(256) void k_head_mfma(Img m, const __bf16* __restri
       const int t = tid + q * 256;
       pre[q] = zero8();
       if (t < NCH) {
-        const int px = t / CH, c = t % CH;
+        const int px = t / CG, c = t % CG;
         const int xi = x0 - 1 + px;
-        if (yi <= m.H && xi <= m.W) pre[q] = load8(X3 + img_pix(m, yi, xi) * NET_C + 8 * c);
+        if (yi <= m1.H && xi <= m1.W) pre[q] = load8(U1 + img_pix(m1, yi, xi) * NET_C + 8 * c);
       }
     }
   };
@@ -248,42 +450,55 @@ __global__ __launch_bounds__(256) void k_head_mfma(Img m, const __bf16* __restri
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
       const int dy = kb / 3 - 1, dx = kb % 3 - 1;
-      acc = mfma16(a[kb], sRow[(y + dy + 3) % 3][(xo + 1 + dx) * CH + lg], acc);
+      acc = mfma16(a[kb], sRow[(y + dy + 3) % 3][(xo + 1 + dx) * CG + lg], acc);
     }
     const int x = x0 + xo;
-    if (lg == 0 && x < m.W) {
-      const float v = acc[0] + b4;
-      delta[(int64_t)y * m.W + x] = active(m, X0, y, x, gate) ? v : 0.f;
+    if (lg == 0 && x < m1.W) {
+#pragma unroll
+      for (int s = 0; s < NET_HO; ++s) {
+        const int fy = 2 * y + (s >> 1), fx = 2 * x + (s & 1);
+        if (fy < m0.H && fx < m0.W)
+          delta[(int64_t)fy * m0.W + fx] = active(m1, XS, fy, fx, gate) ? acc[s] + bh[s] : 0.f;
+      }
     }
     __syncthreads();
   }
 }
 
-// G3pre[p][ci] = (sum_tap G4[p - off(tap)] W4[tap][ci]) * (X3[p][ci] > 0)
-__global__ __launch_bounds__(256) void k_head_bwd(Img m, const float* __restrict__ G4, const __bf16* __restrict__ w4,
-                                                  const __bf16* __restrict__ X3, __bf16* __restrict__ G3) {
-  __shared__ float sw[9 * NET_C];
-  for (int t = threadIdx.x; t < 9 * NET_C; t += 256) sw[t] = bf(w4[t]);
+// GU1[p][ci] = (sum_tap sum_s G4[p - off(tap)][s] Wh[s][tap][ci]) * (U1[p][ci] > 0)
+__global__ __launch_bounds__(256) void k_head_bwd(Img m1, const float* __restrict__ G4, const __bf16* __restrict__ wh,
+                                                  const __bf16* __restrict__ U1, __bf16* __restrict__ GU1) {
+  __shared__ float sw[NET_HO * 9 * NET_C];
+  for (int t = threadIdx.x; t < NET_HO * 9 * NET_C; t += 256) sw[t] = bf(wh[t]);
   __syncthreads();
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (int64_t)m.H * m.W) return;
-  const int y = (int)(idx / m.W), x = (int)(idx % m.W);
-  float g[9];
+  if (idx >= (int64_t)m1.H * m1.W) return;
+  const int y = (int)(idx / m1.W), x = (int)(idx % m1.W);
+  float g[9][NET_HO];
 #pragma unroll
-  for (int tap = 0; tap < 9; ++tap) g[tap] = G4[img_pix(m, y - (tap / 3 - 1), x - (tap % 3 - 1))];
-  const int64_t p = img_pix(m, y, x);
+  for (int tap = 0; tap < 9; ++tap) {
+    const float4 v =
+        *reinterpret_cast<const float4*>(G4 + img_pix(m1, y - (tap / 3 - 1), x - (tap % 3 - 1)) * NET_HO);
+    g[tap][0] = v.x;
+    g[tap][1] = v.y;
+    g[tap][2] = v.z;
+    g[tap][3] = v.w;
+  }
+  const int64_t p = img_pix(m1, y, x);
 #pragma unroll
-  for (int q = 0; q < NET_C / 8; ++q) {
-    const bf16x8 xm = load8(X3 + p * NET_C + 8 * q);
+  for (int q = 0; q < CG; ++q) {
+    const bf16x8 um = load8(U1 + p * NET_C + 8 * q);
     bf16x8 o;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       float s = 0.f;
 #pragma unroll
-      for (int tap = 0; tap < 9; ++tap) s += g[tap] * sw[tap * NET_C + 8 * q + j];
-      o[j] = (__bf16)(bf(xm[j]) > 0.f ? s : 0.f);
+      for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int h = 0; h < NET_HO; ++h) s += g[tap][h] * sw[(h * 9 + tap) * NET_C + 8 * q + j];
+      o[j] = (__bf16)(bf(um[j]) > 0.f ? s : 0.f);
     }
-    *reinterpret_cast<uint4*>(G3 + p * NET_C + 8 * q) = __builtin_bit_cast(uint4, o);
+    store8(GU1 + p * NET_C + 8 * q, o);
   }
 }
 
@@ -291,15 +506,12 @@ constexpr int WG_RB = 4;       // rows per LDS stage
 constexpr int WG_ROWS = 32;    // rows per block
 constexpr int WG_GS = NET_C + 2;  // LDS stride of a G pixel (bank spread)
 
-template <int CIN>
-constexpr int wg_ntiles() {
-  return (9 * CIN + 1 + 15) / 16;
-}
+constexpr int CIN = NET_C;
+constexpr int wg_ntiles() { return (9 * CIN + 1 + 15) / 16; }
 
-template <int CIN>
 __global__ __launch_bounds__(256) void k_conv_wgrad(Img m, const __bf16* __restrict__ G, const __bf16* __restrict__ X,
                                                     float* __restrict__ part) {
-  constexpr int NT = wg_ntiles<CIN>();
+  constexpr int NT = wg_ntiles();
   constexpr int NTW = (NT + 3) / 4;  // N-tiles per wave
   constexpr int XS = CIN + 2;
   __shared__ __bf16 sG[WG_RB * 64 * WG_GS];
@@ -383,32 +595,37 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Img m, const __bf16* __restr
   }
 }
 
-constexpr int HW_PPB = 4096;  // pixels per block of the head weight gradient
+constexpr int HW_PPB = 4096;  // level-1 pixels per block of the head weight gradient
+constexpr int HW_N = 9 * NET_C + 1;  // (tap, ci) and the bias, per head output
 
-__global__ __launch_bounds__(256) void k_head_wgrad(Img m, const float* __restrict__ G4, const __bf16* __restrict__ X3,
+// head weight gradient partials: thread o < 289 of a block sums, over its
+// pixels, G4[p][s] U1[p + off(tap)][ci] (o = 32 tap + ci) or G4[p][s] (o = 288)
+// for all four outputs s; part[block][s][o]
+__global__ __launch_bounds__(256) void k_head_wgrad(Img m1, const float* __restrict__ G4, const __bf16* __restrict__ U1,
                                                     float* __restrict__ part) {
-  const int64_t npx = (int64_t)m.H * m.W;
+  const int64_t npx = (int64_t)m1.H * m1.W;
   const int64_t p0 = (int64_t)blockIdx.x * HW_PPB, p1 = min(p0 + HW_PPB, npx);
-  for (int o = threadIdx.x; o < 9 * NET_C + 1; o += 256) {
-    float s = 0.f;
-    if (o < 9 * NET_C) {
-      const int tap = o / NET_C, ci = o % NET_C;
-      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-      for (int64_t q = p0; q < p1; ++q) {
-        const int y = (int)(q / m.W), x = (int)(q % m.W);
-        s += G4[img_pix(m, y, x)] * bf(X3[img_pix(m, y + dy, x + dx) * NET_C + ci]);
-      }
-    } else {
-      for (int64_t q = p0; q < p1; ++q) s += G4[img_pix(m, (int)(q / m.W), (int)(q % m.W))];
+  for (int o = threadIdx.x; o < HW_N; o += 256) {
+    float s[NET_HO] = {0.f, 0.f, 0.f, 0.f};
+    const int tap = o / NET_C, ci = o % NET_C;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    for (int64_t q = p0; q < p1; ++q) {
+      const int y = (int)(q / m1.W), x = (int)(q % m1.W);
+      const float4 g = *reinterpret_cast<const float4*>(G4 + img_pix(m1, y, x) * NET_HO);
+      const float xv = o < 9 * NET_C ? bf(U1[img_pix(m1, y + dy, x + dx) * NET_C + ci]) : 1.f;
+      s[0] += g.x * xv;
+      s[1] += g.y * xv;
+      s[2] += g.z * xv;
+      s[3] += g.w * xv;
     }
-    part[(int64_t)blockIdx.x * (9 * NET_C + 1) + o] = s;
+#pragma unroll
+    for (int h = 0; h < NET_HO; ++h) part[((int64_t)blockIdx.x * NET_HO + h) * HW_N + o] = s[h];
   }
 }
 
-template <int CIN>
 __global__ void k_reduce_conv_grad(int nblocks, const float* __restrict__ part, float* __restrict__ gW,
                                    float* __restrict__ gB) {
-  constexpr int NPAD = wg_ntiles<CIN>() * 16, NK = 9 * CIN;
+  constexpr int NPAD = wg_ntiles() * 16, NK = 9 * NET_C;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= NET_C * (NK + 1)) return;
   const int co = j / (NK + 1), n = j % (NK + 1);
@@ -422,10 +639,14 @@ __global__ void k_reduce_conv_grad(int nblocks, const float* __restrict__ part, 
 
 __global__ void k_reduce_head_grad(int nblocks, const float* __restrict__ part, float* __restrict__ grad) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= 9 * NET_C + 1) return;
+  if (j >= NET_HO * HW_N) return;
+  const int h = j / HW_N, o = j % HW_N;
   float s = 0.f;
-  for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * (9 * NET_C + 1) + j];
-  grad[NP_W4 + j] = s;
+  for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * NET_HO * HW_N + j];
+  if (o < 9 * NET_C)
+    grad[NP_WH + h * 9 * NET_C + o] = s;
+  else
+    grad[NP_BH + h] = s;
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
@@ -442,21 +663,17 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 
 __global__ void k_pack_weights(const float* __restrict__ P, NetWeights w) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  // w1p: [32][96]
-  if (i < NET_C * 96) {
-    const int co = i / 96, k = i % 96;
-    w.w1p[i] = (__bf16)(k < 9 * NET_C0 ? P[NP_W1 + co * 9 * NET_C0 + k] : 0.f);
-  }
   if (i < NET_C * 9 * NET_C) {
-    w.w2p[i] = (__bf16)P[NP_W2 + i];
-    w.w3p[i] = (__bf16)P[NP_W3 + i];
     // flipped: out channel ci, tap t', in channel co  <-  W[co][8 - t'][ci]
     const int ci = i / (9 * NET_C), tp = (i / NET_C) % 9, co = i % NET_C;
     const int src = co * 9 * NET_C + (8 - tp) * NET_C + ci;
-    w.w2f[i] = (__bf16)P[NP_W2 + src];
-    w.w3f[i] = (__bf16)P[NP_W3 + src];
+#pragma unroll
+    for (int l = 1; l <= NET_CONVS; ++l) {
+      w.wp[l][i] = (__bf16)P[np_w(l) + i];
+      if (l >= 2) w.wf[l][i] = (__bf16)P[np_w(l) + src];
+    }
   }
-  if (i < 9 * NET_C) w.w4[i] = (__bf16)P[NP_W4 + i];
+  if (i < NET_HO * 9 * NET_C) w.wh[i] = (__bf16)P[NP_WH + i];
 }
 
 __global__ void k_fill_bf16(__bf16* p, int64_t n) {
@@ -464,25 +681,18 @@ __global__ void k_fill_bf16(__bf16* p, int64_t n) {
   if (i < n) p[i] = (__bf16)0.f;
 }
 
-__global__ void k_sum_loss(const double* __restrict__ part, int n, double* __restrict__ out) {
-  __shared__ double s[256];
-  double a = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) a += part[i];
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = s[0];
-}
-
 __device__ __forceinline__ float finite_or_zero(double v) { return isfinite(v) ? (float)v : 0.f; }
 
-__global__ void k_features(FeatIn f, Img m, __bf16* __restrict__ X0) {
+// The 8 features of a domain cell: T^n / max|T^n|; the previous increment
+// rescaled to this dt, / E; J / J_ref - 1; log(1 + dt abar_xi g11) and
+// log(1 + dt abar_eta g22) (the face diffusion numbers); dt r(T^n) / E; 1;
+// the largest |feature 5| over the 3x3 neighbourhood.  Stored in the
+// level-1 block layout (cell
+// (y, x) is channel group 2 (y & 1) + (x & 1) of level-1 pixel (y/2, x/2))
+__global__ void k_features(FeatIn f, Img m0, Img m1, __bf16* __restrict__ XS) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)m.H * m.W) return;
-  const int y = (int)(idx / m.W), x = (int)(idx % m.W);
+  if (idx >= (int64_t)m0.H * m0.W) return;
+  const int y = (int)(idx / m0.W), x = (int)(idx % m0.W);
   const Geo& g = f.g;
   const int i = g.lo[0] + x, j = g.lo[1] + y;
   const int64_t c = at(g, i, j, 0);
@@ -499,8 +709,18 @@ __global__ void k_features(FeatIn f, Img m, __bf16* __restrict__ X0) {
   o[4] = (__bf16)finite_or_zero(log1p(fmax(0.0, f.dt * f.g22[c] * ay)));
   o[5] = (__bf16)finite_or_zero(f.dt * (double)f.r32[c] * ie);
   o[6] = (__bf16)1.f;
-  o[7] = (__bf16)0.f;
-  *reinterpret_cast<uint4*>(X0 + img_pix(m, y, x) * NET_C0) = __builtin_bit_cast(uint4, o);
+  // feature 7: the largest |feature 5| over the 3x3 neighbourhood (cells
+  // outside the domain count 0), each rounded as feature 5 is
+  float act = 0.f;
+  for (int tap = 0; tap < 9; ++tap) {
+    const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+    if (yy >= 0 && yy < m0.H && xx >= 0 && xx < m0.W) {
+      const __bf16 v = (__bf16)finite_or_zero(f.dt * (double)f.r32[at(g, g.lo[0] + xx, g.lo[1] + yy, 0)] * ie);
+      act = fmaxf(act, fabsf(bf(v)));
+    }
+  }
+  o[7] = (__bf16)act;
+  store8(XS + img_pix(m1, y >> 1, x >> 1) * NET_C + 8 * (2 * (y & 1) + (x & 1)), o);
 }
 
 __global__ void k_target(Geo g, Img m, const double* __restrict__ unew, const double* __restrict__ uold,
@@ -562,64 +782,103 @@ __global__ void k_max_final(const double* __restrict__ part, int n, double* __re
 
 dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+dim3 conv_grid(Img m) { return dim3((m.W + 63) / 64, (m.H + CONV_RB - 1) / CONV_RB); }
+
 }  // namespace
 
-void launch_conv3x3(int cin, int mode, Img m, const __bf16* X, const __bf16* Wp, const float* bias,
-                    const __bf16* mask, __bf16* Y, hipStream_t s) {
-  const dim3 grid((m.W + 63) / 64, (m.H + CONV_RB - 1) / CONV_RB);
-  if (cin == NET_C0)
-    hipLaunchKernelGGL((k_conv3x3<NET_C0, 0>), grid, dim3(256), 0, s, m, X, Wp, bias, mask, Y);
-  else if (mode == 0)
-    hipLaunchKernelGGL((k_conv3x3<NET_C, 0>), grid, dim3(256), 0, s, m, X, Wp, bias, mask, Y);
+void launch_conv3x3(int mode, Img m, const __bf16* X, const __bf16* Wp, const float* bias, const __bf16* mask,
+                    __bf16* Y, hipStream_t s) {
+  if (mode == 0)
+    hipLaunchKernelGGL(k_conv3x3<0>, conv_grid(m), dim3(256), 0, s, m, X, Wp, bias, mask, Y);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_conv3x3<1>, conv_grid(m), dim3(256), 0, s, m, X, Wp, bias, mask, Y);
   else
-    hipLaunchKernelGGL((k_conv3x3<NET_C, 1>), grid, dim3(256), 0, s, m, X, Wp, bias, mask, Y);
+    hipLaunchKernelGGL(k_conv3x3<2>, conv_grid(m), dim3(256), 0, s, m, X, Wp, bias, mask, Y);
 }
 
-void launch_head_infer(Img m, const __bf16* X3, const __bf16* w4, const float* params, const __bf16* X0, float gate,
+void launch_pool(Img m, const __bf16* X, __bf16* P, hipStream_t s) {
+  const Img mc = img_half(m);
+  hipLaunchKernelGGL(k_pool, grid1((int64_t)mc.H * mc.W * CG), dim3(256), 0, s, m, mc, X, P);
+}
+
+void launch_up_add(Img m, const __bf16* C, const __bf16* S, __bf16* A, hipStream_t s) {
+  hipLaunchKernelGGL(k_up_add, grid1((int64_t)m.H * m.W * CG), dim3(256), 0, s, m, img_half(m), C, S, A);
+}
+
+void launch_grad_skip(Img m, const __bf16* GA, const __bf16* GP, const __bf16* E, __bf16* GE, hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_skip, grid1((int64_t)m.H * m.W * CG), dim3(256), 0, s, m, img_half(m), GA, GP, E, GE);
+}
+
+void launch_grad_up(Img m, const __bf16* GA, const __bf16* U, __bf16* GU, hipStream_t s) {
+  const Img mc = img_half(m);
+  hipLaunchKernelGGL(k_grad_up, grid1((int64_t)mc.H * mc.W * CG), dim3(256), 0, s, m, mc, GA, U, GU);
+}
+
+void launch_s2d(Img m0, const __bf16* X0, __bf16* XS, hipStream_t s) {
+  hipLaunchKernelGGL(k_s2d, grid1((int64_t)m0.H * m0.W), dim3(256), 0, s, m0, img_half(m0), X0, XS);
+}
+
+void launch_head_infer(Img m0, const __bf16* U1, const __bf16* wh, const float* params, const __bf16* XS, float gate,
                        float* delta, hipStream_t s) {
-  const dim3 grid((m.W + 63) / 64, (m.H + CONV_RB - 1) / CONV_RB);
-  hipLaunchKernelGGL(k_head_mfma, grid, dim3(256), 0, s, m, X3, w4, params, X0, gate, delta);
-}
-void launch_head_fwd(Img m, const __bf16* X3, const __bf16* w4, const float* params, const __bf16* X0, float gate,
-                     float* delta, const float* target, float* G4, double* loss_part, int* nparts, hipStream_t s) {
-  const int64_t npx = (int64_t)m.H * m.W;
-  hipLaunchKernelGGL(k_head_fwd, grid1(npx), dim3(256), 0, s, m, X3, w4, params, X0, gate, delta, target, G4,
-                     loss_part);
-  if (nparts) *nparts = (int)((npx + 255) / 256);
+  const Img m1 = img_half(m0);
+  hipLaunchKernelGGL(k_head_mfma, conv_grid(m1), dim3(256), 0, s, m0, m1, U1, wh, params, XS, gate, delta);
 }
 
-void launch_head_bwd(Img m, const float* G4, const __bf16* w4, const __bf16* X3, __bf16* G3, hipStream_t s) {
-  hipLaunchKernelGGL(k_head_bwd, grid1((int64_t)m.H * m.W), dim3(256), 0, s, m, G4, w4, X3, G3);
+void launch_head_fwd(Img m0, const __bf16* U1, const __bf16* wh, const float* params, const __bf16* XS, float gate,
+                     float* delta, hipStream_t s) {
+  const Img m1 = img_half(m0);
+  hipLaunchKernelGGL(k_head_fwd, grid1((int64_t)m1.H * m1.W), dim3(256), 0, s, m0, m1, U1, wh, params, XS, gate,
+                     delta);
 }
 
-int conv_wgrad_npad(int cin) { return cin == NET_C0 ? wg_ntiles<NET_C0>() * 16 : wg_ntiles<NET_C>() * 16; }
+int loss_parts(Img m0) { return (int)(((int64_t)m0.H * m0.W + 255) / 256); }
 
-int launch_conv_wgrad(int cin, Img m, const __bf16* G, const __bf16* X, float* part, hipStream_t s) {
-  const dim3 grid((m.W + 63) / 64, (m.H + WG_ROWS - 1) / WG_ROWS);
-  if (cin == NET_C0)
-    hipLaunchKernelGGL((k_conv_wgrad<NET_C0>), grid, dim3(256), 0, s, m, G, X, part);
+void launch_loss(int kind, Img m0, const __bf16* XS, const float* delta, const float* target, float gate, float* re,
+                 double* part, double* sums, double* loss, float* G4, hipStream_t s) {
+  const Img m1 = img_half(m0);
+  const int np = loss_parts(m0);
+  if (kind == 1)
+    hipLaunchKernelGGL(k_res_fwd<true>, dim3(np), dim3(256), 0, s, m0, m1, XS, delta, target, re, part);
   else
-    hipLaunchKernelGGL((k_conv_wgrad<NET_C>), grid, dim3(256), 0, s, m, G, X, part);
+    hipLaunchKernelGGL(k_res_fwd<false>, dim3(np), dim3(256), 0, s, m0, m1, XS, delta, target, re, part);
+  hipLaunchKernelGGL(k_res_sums, dim3(1), dim3(256), 0, s, (const double*)part, np, sums, loss);
+  if (kind == 1)
+    hipLaunchKernelGGL(k_res_bwd<true>, grid1((int64_t)m1.H * m1.W), dim3(256), 0, s, m0, m1, XS, (const float*)re,
+                       (const double*)sums, gate, G4);
+  else
+    hipLaunchKernelGGL(k_res_bwd<false>, grid1((int64_t)m1.H * m1.W), dim3(256), 0, s, m0, m1, XS,
+                       (const float*)re, (const double*)sums, gate, G4);
+}
+
+void launch_head_bwd(Img m0, const float* G4, const __bf16* wh, const __bf16* U1, __bf16* GU1, hipStream_t s) {
+  const Img m1 = img_half(m0);
+  hipLaunchKernelGGL(k_head_bwd, grid1((int64_t)m1.H * m1.W), dim3(256), 0, s, m1, G4, wh, U1, GU1);
+}
+
+int conv_wgrad_npad() { return wg_ntiles() * 16; }
+
+int conv_wgrad_blocks(Img m) { return ((m.W + 63) / 64) * ((m.H + WG_ROWS - 1) / WG_ROWS); }
+
+int launch_conv_wgrad(Img m, const __bf16* G, const __bf16* X, float* part, hipStream_t s) {
+  const dim3 grid((m.W + 63) / 64, (m.H + WG_ROWS - 1) / WG_ROWS);
+  hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, s, m, G, X, part);
   return (int)(grid.x * grid.y);
 }
 
-int launch_head_wgrad(Img m, const float* G4, const __bf16* X3, float* part, hipStream_t s) {
-  const int nb = (int)(((int64_t)m.H * m.W + HW_PPB - 1) / HW_PPB);
-  hipLaunchKernelGGL(k_head_wgrad, dim3(nb), dim3(256), 0, s, m, G4, X3, part);
+int head_wgrad_blocks(Img m1) { return (int)(((int64_t)m1.H * m1.W + HW_PPB - 1) / HW_PPB); }
+
+int launch_head_wgrad(Img m1, const float* G4, const __bf16* U1, float* part, hipStream_t s) {
+  const int nb = head_wgrad_blocks(m1);
+  hipLaunchKernelGGL(k_head_wgrad, dim3(nb), dim3(256), 0, s, m1, G4, U1, part);
   return nb;
 }
 
-void launch_reduce_conv_grad(int cin, int nblocks, const float* part, float* gW, float* gB, hipStream_t s) {
-  if (cin == NET_C0)
-    hipLaunchKernelGGL((k_reduce_conv_grad<NET_C0>), grid1(NET_C * (9 * NET_C0 + 1)), dim3(256), 0, s, nblocks,
-                       part, gW, gB);
-  else
-    hipLaunchKernelGGL((k_reduce_conv_grad<NET_C>), grid1(NET_C * (9 * NET_C + 1)), dim3(256), 0, s, nblocks, part,
-                       gW, gB);
+void launch_reduce_conv_grad(int nblocks, const float* part, float* gW, float* gB, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_conv_grad, grid1(NET_C * (9 * NET_C + 1)), dim3(256), 0, s, nblocks, part, gW, gB);
 }
 
 void launch_reduce_head_grad(int nblocks, const float* part, float* grad, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_head_grad, grid1(9 * NET_C + 1), dim3(256), 0, s, nblocks, part, grad);
+  hipLaunchKernelGGL(k_reduce_head_grad, grid1(NET_HO * HW_N), dim3(256), 0, s, nblocks, part, grad);
 }
 
 void launch_adam(float* params, const float* grad, float* m1, float* m2, int n, float lr, float b1, float b2,
@@ -636,13 +895,9 @@ void launch_fill_bf16(__bf16* p, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_bf16, grid1(n), dim3(256), 0, s, p, n);
 }
 
-void launch_sum_loss(const double* part, int n, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, s, part, n, out);
-}
-
-void launch_features(const FeatIn& f, __bf16* X0, hipStream_t s) {
-  const Img m{f.g.hi[1] - f.g.lo[1] + 1, f.g.hi[0] - f.g.lo[0] + 1};
-  hipLaunchKernelGGL(k_features, grid1((int64_t)m.H * m.W), dim3(256), 0, s, f, m, X0);
+void launch_features(const FeatIn& f, __bf16* XS, hipStream_t s) {
+  const Img m0{f.g.hi[1] - f.g.lo[1] + 1, f.g.hi[0] - f.g.lo[0] + 1};
+  hipLaunchKernelGGL(k_features, grid1((int64_t)m0.H * m0.W), dim3(256), 0, s, f, m0, img_half(m0), XS);
 }
 
 void launch_target(Geo g, const double* unew, const double* uold, const double* scales, float* tgt, hipStream_t s) {

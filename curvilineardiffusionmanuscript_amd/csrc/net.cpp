@@ -48,25 +48,24 @@ int cdiff_net::init(uint64_t seed) {
   HIP_TRY(alloc((void**)&grad, sizeof(float) * NP_TOTAL));
   HIP_TRY(alloc((void**)&m1, sizeof(float) * NP_TOTAL));
   HIP_TRY(alloc((void**)&m2, sizeof(float) * NP_TOTAL));
-  HIP_TRY(alloc((void**)&w.w1p, sizeof(__bf16) * NET_C * 96));
-  HIP_TRY(alloc((void**)&w.w2p, sizeof(__bf16) * NET_C * 9 * NET_C));
-  HIP_TRY(alloc((void**)&w.w3p, sizeof(__bf16) * NET_C * 9 * NET_C));
-  HIP_TRY(alloc((void**)&w.w2f, sizeof(__bf16) * NET_C * 9 * NET_C));
-  HIP_TRY(alloc((void**)&w.w3f, sizeof(__bf16) * NET_C * 9 * NET_C));
-  HIP_TRY(alloc((void**)&w.w4, sizeof(__bf16) * 9 * NET_C));
+  for (int l = 1; l <= NET_CONVS; ++l) {
+    HIP_TRY(alloc((void**)&w.wp[l], sizeof(__bf16) * NET_C * 9 * NET_C));
+    if (l >= 2) HIP_TRY(alloc((void**)&w.wf[l], sizeof(__bf16) * NET_C * 9 * NET_C));
+  }
+  HIP_TRY(alloc((void**)&w.wh, sizeof(__bf16) * NET_HO * 9 * NET_C));
   w.params = params;
-  // He initialisation of the ReLU layers, a small head: the first predictions
-  // are near zero (the warm start then falls back to T^n)
+  // He initialisation of the ReLU layers (the first layer's fan-in counts the
+  // 24 features of a block that carry data), a small head: the first
+  // predictions are near zero (the warm start then falls back to T^n)
   std::vector<float> p(NP_TOTAL, 0.f);
   std::mt19937_64 rng(seed);
   std::normal_distribution<float> nd(0.f, 1.f);
   auto fill = [&](int off, int n, float sd) {
     for (int i = 0; i < n; ++i) p[off + i] = sd * nd(rng);
   };
-  fill(NP_W1, NET_C * 9 * NET_C0, std::sqrt(2.f / (9 * 6)));
-  fill(NP_W2, NET_C * 9 * NET_C, std::sqrt(2.f / (9 * NET_C)));
-  fill(NP_W3, NET_C * 9 * NET_C, std::sqrt(2.f / (9 * NET_C)));
-  fill(NP_W4, 9 * NET_C, 0.01f * std::sqrt(1.f / (9 * NET_C)));
+  fill(np_w(1), NET_C * 9 * NET_C, std::sqrt(2.f / (9 * 24)));
+  for (int l = 2; l <= NET_CONVS; ++l) fill(np_w(l), NET_C * 9 * NET_C, std::sqrt(2.f / (9 * NET_C)));
+  fill(NP_WH, NET_HO * 9 * NET_C, 0.01f * std::sqrt(1.f / (9 * NET_C)));
   HIP_TRY(hipMemcpyAsync(params, p.data(), sizeof(float) * NP_TOTAL, hipMemcpyHostToDevice, stream));
   launch_pack_weights(params, w, stream);
   LAUNCH_CHECK("k_pack_weights");
@@ -75,7 +74,7 @@ int cdiff_net::init(uint64_t seed) {
 }
 
 int cdiff_net::workspace(Img m) {
-  if (m.H == wimg.H && m.W == wimg.W && X1) return CDIFF_OK;
+  if (m.H == wimg.H && m.W == wimg.W && XS) return CDIFF_OK;
   if (m.H < 1 || m.W < 1) return fail(CDIFF_BAD_ARG, "image size must be positive");
   HIP_TRY(hipStreamSynchronize(stream));
   for (void* p : ws) (void)hipFree(p);
@@ -89,53 +88,81 @@ int cdiff_net::workspace(Img m) {
     }
     return e;
   };
+  const Img lv1 = img_half(m), lv2 = img_half(lv1), lv3 = img_half(lv2);
+  const int64_t n1 = img_pixels(lv1), n2 = img_pixels(lv2), n3 = img_pixels(lv3);
   HIP_TRY(alloc((void**)&X0, sizeof(__bf16) * npx * NET_C0));
-  for (__bf16** b : {&X1, &X2, &X3, &G1, &G2, &G3}) HIP_TRY(alloc((void**)b, sizeof(__bf16) * npx * NET_C));
+  for (__bf16** b : {&XS, &E1, &A6, &U1, &GU1, &GA6, &GE1}) HIP_TRY(alloc((void**)b, sizeof(__bf16) * n1 * NET_C));
+  for (__bf16** b : {&P1, &E2, &A5, &U2, &GU2, &GA5, &GE2, &GP1})
+    HIP_TRY(alloc((void**)b, sizeof(__bf16) * n2 * NET_C));
+  for (__bf16** b : {&P2, &E3, &B3, &GB3, &GE3, &GP2}) HIP_TRY(alloc((void**)b, sizeof(__bf16) * n3 * NET_C));
   HIP_TRY(alloc((void**)&delta, sizeof(float) * ni));
   HIP_TRY(alloc((void**)&tgt, sizeof(float) * ni));
-  HIP_TRY(alloc((void**)&G4, sizeof(float) * npx));
-  HIP_TRY(alloc((void**)&loss_part, sizeof(double) * ((ni + 255) / 256)));
-  // weight-gradient partials: the largest of the conv layers and the head
-  const int64_t conv_blocks = (int64_t)((m.W + 63) / 64) * ((m.H + 31) / 32);
-  const int64_t conv_part = conv_blocks * NET_C * conv_wgrad_npad(NET_C);
-  const int64_t head_part = ((ni + 4095) / 4096) * (9 * NET_C + 1);
+  HIP_TRY(alloc((void**)&G4, sizeof(float) * n1 * NET_HO));
+  HIP_TRY(alloc((void**)&loss_part, sizeof(double) * 2 * loss_parts(m)));
+  HIP_TRY(alloc((void**)&loss_sums, sizeof(double) * 2));
+  HIP_TRY(alloc((void**)&res, sizeof(float) * ni));
+  // weight-gradient partials: the largest of the conv layers (level 1) and the head
+  const int64_t conv_part = (int64_t)conv_wgrad_blocks(lv1) * NET_C * conv_wgrad_npad();
+  const int64_t head_part = (int64_t)head_wgrad_blocks(lv1) * NET_HO * (9 * NET_C + 1);
   HIP_TRY(alloc((void**)&part, sizeof(float) * std::max(conv_part, head_part)));
   HIP_TRY(hipStreamSynchronize(stream));
   wimg = m;
   return CDIFF_OK;
 }
 
-int cdiff_net::forward(const __bf16* X0in, hipStream_t s) {
-  const Img m = wimg;
-  launch_conv3x3(NET_C0, 0, m, X0in, w.w1p, params + NP_B1, nullptr, X1, s);
-  launch_conv3x3(NET_C, 0, m, X1, w.w2p, params + NP_B2, nullptr, X2, s);
-  launch_conv3x3(NET_C, 0, m, X2, w.w3p, params + NP_B3, nullptr, X3, s);
-  launch_head_infer(m, X3, w.w4, params, X0in, gate, delta, s);
+// the encoder-decoder up to U1 (learn.h)
+static void encode_decode(cdiff_net& n, const __bf16* XSin, hipStream_t s) {
+  const Img lv1 = img_half(n.wimg), lv2 = img_half(lv1), lv3 = img_half(lv2);
+  const NetWeights& w = n.w;
+  const float* P = n.params;
+  launch_conv3x3(0, lv1, XSin, w.wp[1], P + np_b(1), nullptr, n.E1, s);
+  launch_pool(lv1, n.E1, n.P1, s);
+  launch_conv3x3(0, lv2, n.P1, w.wp[2], P + np_b(2), nullptr, n.E2, s);
+  launch_pool(lv2, n.E2, n.P2, s);
+  launch_conv3x3(0, lv3, n.P2, w.wp[3], P + np_b(3), nullptr, n.E3, s);
+  launch_conv3x3(0, lv3, n.E3, w.wp[4], P + np_b(4), nullptr, n.B3, s);
+  launch_up_add(lv2, n.B3, n.E2, n.A5, s);
+  launch_conv3x3(0, lv2, n.A5, w.wp[5], P + np_b(5), nullptr, n.U2, s);
+  launch_up_add(lv1, n.U2, n.E1, n.A6, s);
+  launch_conv3x3(0, lv1, n.A6, w.wp[6], P + np_b(6), nullptr, n.U1, s);
+}
+
+int cdiff_net::forward(const __bf16* XSin, hipStream_t s) {
+  encode_decode(*this, XSin, s);
+  launch_head_infer(wimg, U1, w.wh, params, XSin, gate, delta, s);
   LAUNCH_CHECK("network forward");
   return CDIFF_OK;
 }
 
-int cdiff_net::backward(const __bf16* X0in, const float* target, double* loss_dev, hipStream_t s) {
-  const Img m = wimg;
-  launch_conv3x3(NET_C0, 0, m, X0in, w.w1p, params + NP_B1, nullptr, X1, s);
-  launch_conv3x3(NET_C, 0, m, X1, w.w2p, params + NP_B2, nullptr, X2, s);
-  launch_conv3x3(NET_C, 0, m, X2, w.w3p, params + NP_B3, nullptr, X3, s);
-  int nparts = 0;
-  launch_head_fwd(m, X3, w.w4, params, X0in, gate, delta, target, G4, loss_part, &nparts, s);
-  launch_sum_loss(loss_part, nparts, loss_dev, s);
-  // backward data: head, layer 3, layer 2 (with the ReLU derivatives)
-  launch_head_bwd(m, G4, w.w4, X3, G3, s);
-  launch_conv3x3(NET_C, 1, m, G3, w.w3f, nullptr, X2, G2, s);
-  launch_conv3x3(NET_C, 1, m, G2, w.w2f, nullptr, X1, G1, s);
+int cdiff_net::backward(const __bf16* XSin, const float* target, double* loss_dev, hipStream_t s) {
+  const Img lv0 = wimg, lv1 = img_half(lv0), lv2 = img_half(lv1), lv3 = img_half(lv2);
+  encode_decode(*this, XSin, s);
+  launch_head_fwd(lv0, U1, w.wh, params, XSin, gate, delta, s);
+  launch_loss(loss_kind, lv0, XSin, delta, target, gate, res, loss_part, loss_sums, loss_dev, G4, s);
+  // backward data (the ReLU derivatives applied where the skips join)
+  launch_head_bwd(lv0, G4, w.wh, U1, GU1, s);
+  launch_conv3x3(2, lv1, GU1, w.wf[6], nullptr, nullptr, GA6, s);
+  launch_grad_up(lv1, GA6, U2, GU2, s);
+  launch_conv3x3(2, lv2, GU2, w.wf[5], nullptr, nullptr, GA5, s);
+  launch_grad_up(lv2, GA5, B3, GB3, s);
+  launch_conv3x3(1, lv3, GB3, w.wf[4], nullptr, E3, GE3, s);
+  launch_conv3x3(2, lv3, GE3, w.wf[3], nullptr, nullptr, GP2, s);
+  launch_grad_skip(lv2, GA5, GP2, E2, GE2, s);
+  launch_conv3x3(2, lv2, GE2, w.wf[2], nullptr, nullptr, GP1, s);
+  launch_grad_skip(lv1, GA6, GP1, E1, GE1, s);
   LAUNCH_CHECK("network backward data");
-  // weight gradients
-  int nb = launch_conv_wgrad(NET_C, m, G3, X2, part, s);
-  launch_reduce_conv_grad(NET_C, nb, part, grad + NP_W3, grad + NP_B3, s);
-  nb = launch_conv_wgrad(NET_C, m, G2, X1, part, s);
-  launch_reduce_conv_grad(NET_C, nb, part, grad + NP_W2, grad + NP_B2, s);
-  nb = launch_conv_wgrad(NET_C0, m, G1, X0in, part, s);
-  launch_reduce_conv_grad(NET_C0, nb, part, grad + NP_W1, grad + NP_B1, s);
-  nb = launch_head_wgrad(m, G4, X3, part, s);
+  // weight gradients: each layer's pre-activation gradient against its input
+  const struct {
+    int l;
+    Img m;
+    const __bf16 *G, *X;
+  } layers[NET_CONVS] = {{6, lv1, GU1, A6}, {5, lv2, GU2, A5}, {4, lv3, GB3, E3},
+                         {3, lv3, GE3, P2}, {2, lv2, GE2, P1}, {1, lv1, GE1, XSin}};
+  for (const auto& L : layers) {
+    const int nb = launch_conv_wgrad(L.m, L.G, L.X, part, s);
+    launch_reduce_conv_grad(nb, part, grad + np_w(L.l), grad + np_b(L.l), s);
+  }
+  const int nb = launch_head_wgrad(lv1, G4, U1, part, s);
   launch_reduce_head_grad(nb, part, grad, s);
   LAUNCH_CHECK("network weight gradients");
   return CDIFF_OK;
@@ -251,9 +278,8 @@ int cdiff_net_train(cdiff_net* n, int nsamples, int epochs, float lr, uint64_t s
   if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
   if (e != hipSuccess) return n->hip_fail(e, "train");
   if (loss) {
-    const double npx = (double)n->dimg.H * n->dimg.W;
     double s = 0.0;
-    for (double v : l) s += v / npx;
+    for (double v : l) s += v;
     *loss = s / nsamples;
   }
   return CDIFF_OK;
@@ -274,7 +300,7 @@ int cdiff_net_eval(cdiff_net* n, int slot, float* delta, double* loss) {
     e = hipMemcpyAsync(delta, n->delta, sizeof(float) * ni, hipMemcpyDeviceToHost, n->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
   if (e != hipSuccess) return n->hip_fail(e, "eval");
-  if (loss) *loss = l / (double)ni;
+  if (loss) *loss = l;
   return CDIFF_OK;
 }
 
@@ -289,10 +315,11 @@ int cdiff_net_loss_grad(cdiff_net* n, int height, int width, const uint16_t* fea
                                 n->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(n->tgt, target, sizeof(float) * ni, hipMemcpyHostToDevice, n->stream);
   if (e != hipSuccess) return n->hip_fail(e, "upload");
+  launch_s2d(m, n->X0, n->XS, n->stream);
   double* lsum = nullptr;
   e = hipMalloc((void**)&lsum, sizeof(double));
   if (e != hipSuccess) return n->hip_fail(e, "hipMalloc");
-  rc = n->backward(n->X0, n->tgt, lsum, n->stream);
+  rc = n->backward(n->XS, n->tgt, lsum, n->stream);
   double l = 0.0;
   if (!rc) {
     e = hipMemcpyAsync(&l, lsum, sizeof(double), hipMemcpyDeviceToHost, n->stream);
@@ -302,7 +329,7 @@ int cdiff_net_loss_grad(cdiff_net* n, int height, int width, const uint16_t* fea
     if (e != hipSuccess) rc = n->hip_fail(e, "download");
   }
   (void)hipFree(lsum);
-  if (!rc && loss) *loss = l / (double)ni;
+  if (!rc && loss) *loss = l;
   return rc;
 }
 
@@ -315,7 +342,8 @@ int cdiff_net_predict(cdiff_net* n, int height, int width, const uint16_t* featu
   hipError_t e = hipMemcpyAsync(n->X0, features, sizeof(uint16_t) * img_pixels(m) * NET_C0, hipMemcpyHostToDevice,
                                 n->stream);
   if (e != hipSuccess) return n->hip_fail(e, "upload");
-  if ((rc = n->forward(n->X0, n->stream))) return rc;
+  launch_s2d(m, n->X0, n->XS, n->stream);
+  if ((rc = n->forward(n->XS, n->stream))) return rc;
   e = hipMemcpyAsync(delta, n->delta, sizeof(float) * ni, hipMemcpyDeviceToHost, n->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
   if (e != hipSuccess) return n->hip_fail(e, "download");
@@ -323,6 +351,12 @@ int cdiff_net_predict(cdiff_net* n, int height, int width, const uint16_t* featu
 }
 
 const char* cdiff_net_last_error(const cdiff_net* n) { return n ? n->err.c_str() : "null network"; }
+
+int cdiff_net_set_loss(cdiff_net* n, int kind) {
+  if (!n || (kind != NET_LOSS_MSE && kind != NET_LOSS_RESIDUAL)) return CDIFF_BAD_ARG;
+  n->loss_kind = kind;
+  return CDIFF_OK;
+}
 
 int cdiff_net_set_gate(cdiff_net* n, float gate) {
   if (!n || !(gate >= 0.f)) return CDIFF_BAD_ARG;

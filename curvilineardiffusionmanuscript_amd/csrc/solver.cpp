@@ -518,8 +518,8 @@ int cdiff_solver::step(double dt, const cdiff_opts& o, cdiff_stats* stats, doubl
     }
     int rc = net->workspace(Img{g.hi[1] - g.lo[1] + 1, g.hi[0] - g.lo[0] + 1});
     if (rc) return fail(rc, net->err);
-    if ((rc = learn_features(dt, net->X0, learn_ts))) return rc;
-    if ((rc = net->forward(net->X0, stream))) return fail(rc, net->err);
+    if ((rc = learn_features(dt, net->XS, learn_ts))) return rc;
+    if ((rc = net->forward(net->XS, stream))) return fail(rc, net->err);
     warm_pending = true;
     // the feature pass evaluated the residual of T^n (u = u_prev = T^n, BC
     // ghosts by the pass's own rule) with the 2D outer kernel; nothing before

@@ -86,18 +86,17 @@ namespace {
 
 // Work in units of one FP32 correction sweep at the same mesh size, from
 // measurements at 4096^2 on MI355X (all three scale with the cell count): an
-// FP64 APT iteration ~7.4 sweeps (~390 vs 53 us), a fused outer pass of the
-// defect correction ~7.4 sweeps, and one consultation of the network
-// (features, inference, the FP64 residual of the predicted start) ~55 sweeps
-// (~2.9 ms)
-constexpr double WORK_FP64_ITER = 7.4, WORK_OUTER_PASS = 7.4, WORK_CONSULT = 55.0;
+// FP64 APT iteration ~9.2 sweeps (~390 vs 42.4 us), a fused outer pass of the
+// defect correction ~9.2 sweeps, and one consultation of the network
+// (features, inference, the FP64 residual of the predicted start) ~40 sweeps
+// (~1.7 ms)
+constexpr double WORK_FP64_ITER = 9.2, WORK_OUTER_PASS = 9.2, WORK_CONSULT = 40.0;
 
 }  // namespace
 
 // after a consultation that did not pay, the network sits out 15, 31, 63,
-// ... (at most 255) steps before it is consulted again: where it never pays
-// (the mixed-precision hot disk) one trial costs ~55 sweeps, a few percent of
-// the 15 or more steps it then sits out
+// ... (at most 255) steps before it is consulted again: one trial costs ~40
+// sweeps, a few percent of the 15 or more steps it then sits out
 void cdiff_solver::warm_back_off() {
   ++warm_rejects;
   warm_skip = std::min((16 << std::min(warm_rejects - 1, 4)) - 1, 255);
@@ -133,10 +132,10 @@ void cdiff_solver::warm_account(const cdiff_stats& st, bool dc) {
 // Whether a consultation is expected to pay: the last step's work per decade
 // times the decades a prediction is expected to take off (1 before the first
 // used prediction; then what the last one took off) against its cost.  On the
-// FP64 APT a decade costs ~100-200 sweep units at 4096^2 and a consultation
+// FP64 APT a decade costs ~100-250 sweep units at 4096^2 and a consultation
 // pays; the mixed-precision step needs ~15-30 units per decade, so a
-// prediction would have to take 2-4 decades off the start, more than the
-// BF16 network's 0.7-1.4.
+// prediction would have to take 1.5-3 decades off the start, more than the
+// BF16 network's ~1.
 bool cdiff_solver::warm_pays() const {
   const bool pays = work_per_decade * warm_decades > WORK_CONSULT;
   if (tune.verbose)

@@ -1,4 +1,4 @@
-"""The learned warm start: a small BF16 convolutional network, trained on the
+"""The learned warm start: a 4-level BF16 convolutional encoder-decoder, trained on the
 GPU by the library's own kernels, whose prediction of the next time level
 starts the FP32 defect-correction APT (include/cdiff.h, cdiff_net_*).
 
@@ -34,7 +34,8 @@ class NetError(RuntimeError):
 
 
 class WarmStartNet:
-    """A cdiff_net handle: 8 -> 32 -> 32 -> 32 -> 1 channels, 3x3 convolutions."""
+    """A cdiff_net handle: the 4-level encoder-decoder of include/cdiff.h (32
+    channels per level, 3x3 convolutions, 56,644 parameters)."""
 
     def __init__(self, seed: int = 0, gate: float = 1e-5):
         lib = L.load()
@@ -104,6 +105,14 @@ class WarmStartNet:
         return net
 
     # -- training ---------------------------------------------------------------
+    LOSSES = {"mse": 0, "residual": 1}
+
+    def set_loss(self, kind: str):
+        """Training loss: "mse" (mean squared error of the normalised
+        increment) or "residual" (squared relative residual of the predicted
+        start, to first order)."""
+        self._check(L.load().cdiff_net_set_loss(self._h, self.LOSSES[kind]))
+
     def dataset(self, height: int, width: int, capacity: int):
         self._check(L.load().cdiff_net_dataset(self._h, int(height), int(width), int(capacity)))
         self.capacity, self.shape, self.count = capacity, (height, width), 0
@@ -249,14 +258,20 @@ def train_main(args):
     got = generate_pairs(net, n=args.crop, full=args.full, nseeds=nseeds, steps=args.steps, dt0=args.dt0,
                          seed0=args.seed * 100000, log=_log if args.verbose else None)
     _log(f"{got} step pairs ({args.crop}^2 windows of the {args.full}^2 grid) in {time.time() - t0:.1f} s")
-    lrs = [args.lr] * args.epochs
+    # mean squared error first, then (optionally) the residual of the
+    # predicted start at a lower rate
+    plan = [("mse", args.lr)] * args.epochs
     if args.epochs > 1:
-        lrs[-1] = args.lr / 10
-    for ep, lr in enumerate(lrs):
+        plan[-1] = ("mse", args.lr / 10)
+    plan += [("residual", args.finetune_lr)] * args.finetune_epochs
+    for ep, (kind, lr) in enumerate(plan):
         t1 = time.time()
+        net.set_loss(kind)
         loss = net.train(epochs=1, lr=lr, seed=args.seed + ep)
-        _log(f"epoch {ep}: lr {lr:.1e}, mean squared error {loss:.4e}, {time.time() - t1:.1f} s")
-    net.save(args.out, pairs=got, crop=args.crop, full=args.full, epochs=args.epochs, final_loss=loss)
+        _log(f"epoch {ep}: {kind} loss, lr {lr:.1e}: {loss:.4e}, {time.time() - t1:.1f} s")
+    net.save(args.out, pairs=got, crop=args.crop, full=args.full, epochs=args.epochs, lr=args.lr,
+             finetune_epochs=args.finetune_epochs, finetune_lr=args.finetune_lr, final_loss=loss,
+             final_loss_kind=plan[-1][0])
     _log(f"weights -> {args.out}")
 
 
@@ -332,8 +347,11 @@ def main(argv=None):
     t.add_argument("--full", type=int, default=4096)
     t.add_argument("--steps", type=int, default=8, help="time steps per problem (pairs per problem = steps - 1)")
     t.add_argument("--dt0", type=float, default=1e-6)
-    t.add_argument("--epochs", type=int, default=3)
+    t.add_argument("--epochs", type=int, default=8)
     t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--finetune-epochs", type=int, default=16,
+                   help="epochs on the residual of the predicted start after the mean-squared-error epochs")
+    t.add_argument("--finetune-lr", type=float, default=3e-5)
     t.add_argument("--gate", type=float, default=1e-5)
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--out", default=WEIGHTS)

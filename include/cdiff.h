@@ -318,17 +318,20 @@ int cdiff_bench_sweeps(cdiff_solver *h, double dt, int precision, int check, int
 const char *cdiff_last_error(const cdiff_solver *h);
 
 /* ---------------------------------------------------------------------------
- * Learned warm start (no counterpart in the reference).  A 4-layer 3x3
- * convolutional network (8 -> 32 -> 32 -> 32 -> 1 channels, ReLU, BF16
+ * Learned warm start (no counterpart in the reference).  A 4-level
+ * convolutional encoder-decoder (32 channels per level: the 8 features of
+ * each 2x2 block of cells in, 3x3 convolutions with ReLU at 1/2, 1/4 and 1/8
+ * of the grid, 2x2 mean pooling down, nearest up-sampling with additive
+ * skips back up, a 32 -> 4 head giving each cell of a block its value; BF16
  * weights and activations, FP32 accumulation on MFMA) predicts
- * (T^{n+1} - T^n) / max|T^n| per cell from features of the solver state at
- * the start of a step: T^n, the previous increment, the cell Jacobian, the
- * face diffusion numbers dt*alpha_face*g, and the explicit-Euler increment
- * dt*r(T^n).  With a network attached, an FP32_DC step of a 2D mesh without
- * periodic faces starts from T^n + prediction when that start has the smaller
- * FP64 residual, and from T^n otherwise; it always iterates to the same
- * tolerance.  The network is trained on the GPU by its own kernels on step
- * pairs captured from the solver. */
+ * (T^{n+1} - T^n) / (dt max|r(T^n)|) per cell from features of the solver
+ * state at the start of a step: T^n, the previous increment, the cell
+ * Jacobian, the face diffusion numbers dt*alpha_face*g, and the explicit-Euler
+ * increment dt*r(T^n).  With a network attached, an FP32_DC or FP64 step of a
+ * 2D mesh without periodic faces starts from T^n + prediction when that start
+ * has the smaller FP64 residual, and from T^n otherwise; it always iterates
+ * to the same tolerance.  The network is trained on the GPU by its own
+ * kernels on step pairs captured from the solver. */
 typedef struct cdiff_net cdiff_net;
 
 int cdiff_net_num_params(void);
@@ -342,13 +345,20 @@ int cdiff_net_get_params(cdiff_net *net, float *params, int n);
 int cdiff_net_dataset(cdiff_net *net, int height, int width, int capacity);
 /* Capture from a solver whose resident temperature is T^n: stage 0 stores the
  * features of a step of size dt in `slot`; stage 1, after that step, the
- * target (T^{n+1} - T^n) / max|T^n|. */
+ * target (T^{n+1} - T^n) / (dt max|r(T^n)|). */
 int cdiff_net_capture(cdiff_net *net, cdiff_solver *h, int slot, int stage, double dt);
+/* The training loss of a sample (prediction p, target t, e = p - t):
+ * kind 0, the mean squared error |e|^2 / (height*width) (the default);
+ * kind 1, |A e|^2 / |A t|^2 with A v = v + the face diffusion numbers
+ * dt*alpha_face*g times the differences of v across the faces: the squared
+ * ratio of the FP64 residuals of the predicted start and of T^n, to first
+ * order. */
+int cdiff_net_set_loss(cdiff_net *net, int kind);
 /* `epochs` passes of Adam (lr, beta 0.9/0.999) over samples [0, nsamples) in
- * a seeded order; mean squared error of the last epoch in *loss. */
+ * a seeded order; the mean loss of the last epoch in *loss. */
 int cdiff_net_train(cdiff_net *net, int nsamples, int epochs, float lr, uint64_t seed, double *loss);
-/* Prediction and mean squared error of a stored sample (delta: height*width
- * floats, row-major by j, or NULL). */
+/* Prediction and loss of a stored sample (delta: height*width floats,
+ * row-major by j, or NULL). */
 int cdiff_net_eval(cdiff_net *net, int slot, float *delta, double *loss);
 /* Loss and its gradient for host-provided features (bf16 bit patterns,
  * (height+2)*(width+2)*8 with a zero border) and target (height*width):

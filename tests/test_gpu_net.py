@@ -25,18 +25,22 @@ def _blocks(g):
     return {name: g[offs[k] : offs[k + 1]] for k, (name, _) in enumerate(N.SIZES)}
 
 
-@pytest.mark.parametrize("H,W", [(37, 70), (8, 16), (64, 64)])
-def test_loss_gradient_matches_oracle(H, W):
+@pytest.mark.parametrize("H,W,kind", [(37, 70, "mse"), (8, 16, "mse"), (64, 64, "mse"), (37, 70, "residual"),
+                                      (64, 64, "residual")])
+def test_loss_gradient_matches_oracle(H, W, kind):
     p = N.init_params(11)
     X = N.random_features(H, W, 12)
     X[1 : H // 2 + 1, 1 : W // 3 + 1, 5] = 0.0  # an inactive region: prediction and gradient 0 there
+    X[..., 3:5] = np.abs(X[..., 3:5])  # face diffusion numbers expm1(f) >= 0
     Xb = N.bf16_round(X)
     t = np.random.default_rng(13).normal(0, 0.5, (H, W)).astype(np.float32)
     net = LW.WarmStartNet()
     net.params = p
+    net.set_loss(kind)
     loss, g, d = net.loss_grad(N.bf16_bits(X), t)
-    lo, go, do = N.loss_grad(p.astype(np.float64), Xb.astype(np.float64), t.astype(np.float64), bf16=True)
-    print(f"{H}x{W}: loss gpu {loss:.6e} oracle {lo:.6e}")
+    lo, go, do = N.loss_grad(p.astype(np.float64), Xb.astype(np.float64), t.astype(np.float64), bf16=True,
+                             kind=LW.WarmStartNet.LOSSES[kind])
+    print(f"{H}x{W} {kind}: loss gpu {loss:.6e} oracle {lo:.6e}")
     assert abs(loss - lo) <= 1e-2 * lo
     assert np.linalg.norm(d - do) <= 1e-2 * np.linalg.norm(do)
     gb, ob = _blocks(g), _blocks(go)
@@ -80,7 +84,9 @@ def test_training_on_captured_pairs_reduces_loss():
     loss = net.train(epochs=40, lr=2e-3, seed=0)
     _, l1 = net.evaluate(0)
     print(f"loss before {l0:.3e}, last epoch {loss:.3e}, sample 0 after {l1:.3e}")
-    assert l1 < 0.3 * l0
+    # the loss is the squared relative residual of the predicted start: a
+    # third of it is a predicted start with 58 % of the residual of T^n
+    assert l1 < 0.5 * l0
     net.close()
 
 
@@ -125,8 +131,8 @@ def test_warm_started_step_reaches_the_same_tolerance(precision):
 
 def test_network_sits_out_where_it_cannot_pay():
     # the mixed-precision hot disk: a decade of residual reduction costs the
-    # step ~10-30 correction sweeps, so a prediction would have to take 2-5
-    # decades off the start to repay the ~55 sweeps a consultation costs;
+    # step ~13-30 correction sweeps, so a prediction would have to take 1.5-3
+    # decades off the start to repay the ~40 sweeps a consultation costs;
     # the network is never consulted and every step is the plain
     # mixed-precision step, bit for bit.  Consulted anyway (warm_policy 1),
     # the steps still end at the tolerance.
