@@ -88,14 +88,14 @@ namespace {
 // measurements at 4096^2 on MI355X (all three scale with the cell count): an
 // FP64 APT iteration ~9.2 sweeps (~390 vs 42.4 us), a fused outer pass of the
 // defect correction ~9.2 sweeps, and one consultation of the network
-// (features, inference, the FP64 residual of the predicted start) ~40 sweeps
-// (~1.7 ms)
-constexpr double WORK_FP64_ITER = 9.2, WORK_OUTER_PASS = 9.2, WORK_CONSULT = 40.0;
+// (the features with the FP64 residual of T^n, inference ~0.8 ms, the FP64
+// residual of the predicted start) ~50 sweeps (~2.1 ms)
+constexpr double WORK_FP64_ITER = 9.2, WORK_OUTER_PASS = 9.2, WORK_CONSULT = 50.0;
 
 }  // namespace
 
 // after a consultation that did not pay, the network sits out 15, 31, 63,
-// ... (at most 255) steps before it is consulted again: one trial costs ~40
+// ... (at most 255) steps before it is consulted again: one trial costs ~50
 // sweeps, a few percent of the 15 or more steps it then sits out
 void cdiff_solver::warm_back_off() {
   ++warm_rejects;
@@ -133,8 +133,8 @@ void cdiff_solver::warm_account(const cdiff_stats& st, bool dc) {
 // times the decades a prediction is expected to take off (1 before the first
 // used prediction; then what the last one took off) against its cost.  On the
 // FP64 APT a decade costs ~100-250 sweep units at 4096^2 and a consultation
-// pays; the mixed-precision step needs ~15-30 units per decade, so a
-// prediction would have to take 1.5-3 decades off the start, more than the
+// pays; the mixed-precision step needs ~13-30 units per decade, so a
+// prediction would have to take 1.7-4 decades off the start, more than the
 // BF16 network's ~1.
 bool cdiff_solver::warm_pays() const {
   const bool pays = work_per_decade * warm_decades > WORK_CONSULT;

@@ -128,7 +128,7 @@ def test_config5_blobs_4096_with_trained_network(seed):
     # the network's gain on the FP64 APT: predicted starts are used and the
     # pseudo-iterations they save over the ten steps exceed what every
     # consultation cost (features, inference and the residual of the
-    # predicted start: ~1.7 ms at 4096^2, ~4.3 FP64 iterations)
+    # predicted start: ~2.2 ms at 4096^2, ~5.8 FP64 iterations)
     cold = sum(st.niter for st, _ in res["fp64"][2])
     warm = sum(st.niter for st, _ in res["fp64+learned"][2])
     used = sum(int(st.warm_start) for st, _ in res["fp64+learned"][2])
@@ -138,7 +138,7 @@ def test_config5_blobs_4096_with_trained_network(seed):
     print(f"FP64 APT: {cold} iterations cold, {warm} with the network ({used} used of {consulted} consultations); "
           f"{t_cold:.3f} s against {t_warm:.3f} s")
     assert used >= 3
-    assert cold - warm > 4.3 * consulted
+    assert cold - warm > 5.8 * consulted
     # on the mixed-precision step a decade costs too few sweeps for a
     # consultation to pay: the network sits the steps out, which are then the
     # plain mixed-precision steps
