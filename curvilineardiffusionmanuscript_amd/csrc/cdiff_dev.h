@@ -54,6 +54,17 @@ struct Ctrl {
   double ref;       //   and the residual it halved to
 };
 
+// v[f] of a 6-entry table (the per-face BC rules of a Ghost) for a face index
+// known only at run time, by selection: an indexed read would place the
+// kernel argument in scratch memory
+template <typename T>
+__device__ __forceinline__ T pick6(const T (&v)[6], int f) {
+  T r = v[0];
+#pragma unroll
+  for (int q = 1; q < 6; ++q) r = f == q ? v[q] : r;
+  return r;
+}
+
 // Julia `x * isfinite(x)` (Bool strong zero).
 template <typename T>
 __device__ __forceinline__ T strong_zero(T x) {

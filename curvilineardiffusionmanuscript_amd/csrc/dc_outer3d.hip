@@ -44,16 +44,6 @@ __device__ __forceinline__ float gfac3(const Ghost& gh, int f) {
   return m == G_COPY ? 1.0f : (m == G_NEG ? -1.0f : 0.0f);
 }
 
-// gh.mode[f], gh.src[f], gh.val[f] for a face index known only at run time,
-// by selection (an indexed read would place the kernel argument in scratch)
-template <typename T>
-__device__ __forceinline__ T pick6(const T (&v)[6], int f) {
-  T r = v[0];
-#pragma unroll
-  for (int q = 1; q < 6; ++q) r = f == q ? v[q] : r;
-  return r;
-}
-
 // what a region cell needs from memory
 struct CellIn {
   double v;   // u_old of the cell, or of the BC source cell of a face ghost

@@ -84,8 +84,8 @@ __global__ __launch_bounds__(L3T) void sweep3d_col_kernel(Sweep3DArgs<T> A) {
   //      offset, and how its u is formed on domain planes (the BC rule of an
   //      i/j face ghost) -- ghost_u3 with the face known in advance
   enum { C_NONE = 0, C_DOM, C_COPY, C_NEG, C_CONST, C_CELL };
-  // the BC table, copied out of the kernel argument: it is indexed with the
-  // face of a ghost column (once per thread, before the plane loop)
+  // the BC table: selected by the face of a ghost column (once per thread,
+  // before the plane loop)
   const Ghost gh = A.gh;
   int64_t coff[2], soff[2];
   int ck[2];
@@ -107,14 +107,14 @@ __global__ __launch_bounds__(L3T) void sweep3d_col_kernel(Sweep3DArgs<T> A) {
       ck[s] = C_CELL;  // edge column: never a face ghost
     } else {
       const int f = oi ? (i < g.lo[0] ? 0 : 1) : (j < g.lo[1] ? 2 : 3);
-      const int m = gh.mode[f];
+      const int m = pick6(gh.mode, f);
       if (m == G_CONST) {
         ck[s] = C_CONST;
-        cval[s] = (T)gh.val[f];
+        cval[s] = (T)pick6(gh.val, f);
       } else if (m == G_KEEP) {
         ck[s] = C_CELL;
       } else {
-        const int src = gh.src[f];
+        const int src = pick6(gh.src, f);
         ck[s] = m == G_NEG ? C_NEG : C_COPY;
         soff[s] = g.off + (int64_t)(oi ? src : i) + g.pitch * (int64_t)(oj ? src : j);
       }
