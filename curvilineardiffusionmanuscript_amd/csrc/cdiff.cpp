@@ -195,7 +195,7 @@ void cdiff_default_tuning(cdiff_tuning* t) {
   t->dc_depth3d = 2;
   t->dc_planes3d = 0;
   t->dc_planes3d_pair = 0;
-  t->dc_outer_planes3d = 32;
+  t->dc_outer_planes3d = 0;
   t->dc_check = 0;
   t->dc_max_outer = 100;
   t->verbose = 0;
@@ -217,8 +217,8 @@ int cdiff_set_tuning(cdiff_solver* h, const cdiff_tuning* t) {
   if (v.tile_rows != 4 && v.tile_rows != 8 && v.tile_rows != 16) return bad("tile_rows must be 4, 8 or 16");
   if (v.sweep3d < 0 || v.sweep3d > 1) return bad("sweep3d must be 0 or 1");
   if (v.march_rows64 < 0 || v.planes3d < 0 || v.dc_march_rows < 0 || v.dc_planes3d < 0 ||
-      v.dc_planes3d_pair < 0 || v.dc_outer_planes3d < 1)
-    return bad("chunk sizes must be >= 0 (>= 1 for dc_outer_planes3d)");
+      v.dc_planes3d_pair < 0 || v.dc_outer_planes3d < 0)
+    return bad("chunk sizes must be >= 0");
   if (v.seg_min < 1 || v.seg_max < v.seg_min) return bad("need 1 <= seg_min <= seg_max");
   if (v.dc_depth2d != 2 && v.dc_depth2d != 4 && v.dc_depth2d != 6 && v.dc_depth2d != 8)
     return bad("dc_depth2d must be 2, 4, 6 or 8");

@@ -176,7 +176,7 @@ typedef struct {
   int32_t dc_depth3d;       /* iterations per 3D correction launch: 1 or 2 (default) */
   int32_t dc_planes3d;      /* planes per chunk of the one-iteration 3D sweep, 0 = auto */
   int32_t dc_planes3d_pair; /* planes per chunk of the two-iteration 3D sweep, 0 = auto */
-  int32_t dc_outer_planes3d; /* planes per chunk of the fused 3D outer pass: 32 */
+  int32_t dc_outer_planes3d; /* planes per chunk of the fused 3D outer pass, 0 = auto (about one block per CU) */
   int32_t dc_check;         /* correction sweeps between checks, 0 = auto (2D: the depth times max(1, 12 / depth); 3D: 2) */
   int32_t dc_max_outer;     /* correction rounds before the step is redone by the FP64 APT: 100 */
   int32_t verbose;          /* 1: per-round diagnostics of the defect correction on stderr */
