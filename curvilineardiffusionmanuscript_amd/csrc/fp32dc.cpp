@@ -558,7 +558,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
   // fewer sweeps in total (256^3 spots: 48 -> 32 sweeps per step); in 2D
   // a check every 8 to 12 sweeps: every eight-iteration launch checks
   // (4096^2 hot disk: 110 sweeps and 7.64 ms per step, against 125 sweeps and
-  // 8.03 ms checking every second launch), every second six-iteration one
+  // 8.03 ms checking every second launch); at depth 6 every second launch
   const double eta = tune.dc_eta > 0.0 ? tune.dc_eta : (ndim == 3 ? 1e-2 : 1e-3);
   // the last correction only needs to reach the outer tolerance: inner
   // sweeps stop at floor_frac * atol at the latest

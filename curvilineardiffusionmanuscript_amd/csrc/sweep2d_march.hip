@@ -35,9 +35,12 @@
 // Bound.  HBM: at 4096^2 (six iterations per launch, 53 us per iteration) a
 // copy without the arithmetic takes as long, one without the loads 27 us.  Its
 // access pattern -- each wavefront reads 256 bytes of 16 arrays per row, the
-// strips of a chunk in step, 40 chunks apart -- moves ~4.1 TB/s in isolation
+// strips of a chunk in step, chunks apart -- moves ~4.1-4.3 TB/s in isolation
 // (a grid-stride read of the same arrays, 6.2 TB/s); neither more rows in
 // flight per wavefront nor interleaved coefficient records change that much.
+// A launch reads its 76 bytes per cell whatever its depth, so the default
+// runs eight iterations per launch (211 VGPRs, two wavefronts per SIMD; 42.4
+// us per iteration, the access pattern alone ~300 us per launch).
 //
 // Values outside the dependency cone of the strip's cells (the ring's edge
 // lanes and the first rows of a chunk) are computed from incomplete
@@ -326,7 +329,7 @@ void launch_march_pf(const Lin2DArgs& A, int kmax, bool check, hipStream_t s, in
 void launch_sweep2d_march(const Lin2DArgs& A, int kmax, bool check, hipStream_t s) {
   // rows fetched ahead: 1 (4096^2, six iterations per launch: 53 us per
   // iteration; 2 and 3 rows ahead need 182 and 199 VGPRs, two wavefronts per
-  // SIMD, and take 60 us)
+  // SIMD, and take 60 us; eight iterations per launch, the default, 42.4 us)
   launch_march_pf<1>(A, kmax, check, s, lin2d_march_blocks(A.strips, A.chunks));
 }
 
