@@ -126,7 +126,7 @@ def run_gpu(args, mesh, T0, method, net=None):
         st, dt = sol.step_resident(dt, None, 1.0, law, **kw)
     reset()
     dt = args.dt0
-    iters, checks, steps, outer, warm, step_times = [], 0, [], [], 0, []
+    iters, checks, steps, outer, warm, consulted, step_times = [], 0, [], [], 0, 0, []
     t0 = time.perf_counter()  # step_resident returns after the step's work has finished
     for k in range(args.steps):
         st, dt_next = sol.step_resident(dt, None, 1.0, law, **kw)
@@ -134,6 +134,7 @@ def run_gpu(args, mesh, T0, method, net=None):
         outer.append(st.outer)
         checks += st.nchecks
         warm += int(st.warm_start)
+        consulted += int(st.warm_consulted)
         steps.append((dt, st.niter, st.abs_err, st.time))
         step_times.append(st.time)
         dt = dt_next
@@ -146,7 +147,7 @@ def run_gpu(args, mesh, T0, method, net=None):
         mean_ms = sol_bench(sol, steps[-1][0], method, args.sweep_samples)
     sol.close()
     return dict(elapsed=elapsed, iters=iters, outer=outer, checks=checks, T_end=T_end, sweep_ms=mean_ms,
-                abs_err=[s[2] for s in steps], warm_used=warm, step_times=step_times)
+                abs_err=[s[2] for s in steps], warm_used=warm, warm_consulted=consulted, step_times=step_times)
 
 
 # dominant kernel of each method and dimension, with the pseudo-iterations
@@ -321,6 +322,10 @@ def main():
     extra = {}
     if lrn is not None:
         extra["warm_start_used_steps"] = lrn["warm_used"]
+        # the network is consulted only where the saving expected from it
+        # exceeds what a consultation costs (on the mixed-precision step it
+        # does not: the steps are then the plain ones)
+        extra["warm_start_consulted_steps"] = lrn["warm_consulted"]
         extra["without_learned_warm_start"] = {
             "s_per_step": round(s_plain, 6),
             "iters_per_step": round(float(np.mean(g["iters"])), 2),
