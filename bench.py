@@ -14,12 +14,14 @@ kappa = T^3, 8 seeded Gaussian spots on T = 0.01, zero-flux faces, dt0 = 1e-5.
 
 Timed in the same run, from the same state: the mixed-precision step (FP32
 correction sweeps inside the FP64 defect-correction loop) with the learned
-warm start attached (the committed network: each step starts from the
-network's prediction when that has the smaller FP64 residual, else from
-T^n), the same step without the network (config.without_learned_warm_start)
-and the plain FP64 APT.  `value` is the s/time-step of the first, the
-method the metric names (FP32 + learned); in 3D, where the warm start does
-not apply, of the mixed-precision step.
+warm start attached (the committed network, consulted where the saving
+expected from it exceeds a consultation's cost; a consulted step starts from
+the network's prediction when that has the smaller FP64 residual, else from
+T^n), the same step without the network (config.without_learned_warm_start),
+the same step with the network consulted at every step from the second
+(config.learned_consulted_every_step) and the plain FP64 APT.  `value` is the
+s/time-step of the first, the method the metric names (FP32 + learned); in
+3D, where the warm start does not apply, of the mixed-precision step.
 
 Prints one JSON line: s/time-step (lower is better), with
   roofline      the dominant kernel's compulsory HBM bytes per launch (every
@@ -102,7 +104,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(args, mesh, T0, method, net=None):
+def run_gpu(args, mesh, T0, method, net=None, tuning=None):
     """Warm-up steps, then args.steps timed steps of `method` from T0; with
     `net`, the learned warm start is attached (its back-off state reset with
     the solver's)."""
@@ -110,7 +112,7 @@ def run_gpu(args, mesh, T0, method, net=None):
 
     w = WORKLOADS[args.workload]
     law = S.PowerLawConductivity(*w["kappa"])
-    sol = S.PseudoTransientSolver(mesh, device_bcs(w["bcs"]))
+    sol = S.PseudoTransientSolver(mesh, device_bcs(w["bcs"]), tuning=tuning)
     kw = dict(rtol=0.0, atol=args.tol, max_iter=10**7, precision=method)
 
     def reset():
@@ -304,6 +306,9 @@ def main():
 
         net = WarmStartNet.load()
         lrn = run_gpu(args, mesh, T0, args.method, net=net)
+        # for reference: the network consulted at every step from the second
+        # (tuning warm_policy 1), whatever its expected saving
+        lrn_all = run_gpu(args, mesh, T0, args.method, net=net, tuning={"warm_policy": 1})
         net.close()
     n_cells = args.n**ndim
     bpc = BYTES_PER_CELL[(args.method, ndim)]
@@ -326,6 +331,13 @@ def main():
         # exceeds what a consultation costs (on the mixed-precision step it
         # does not: the steps are then the plain ones)
         extra["warm_start_consulted_steps"] = lrn["warm_consulted"]
+        extra["learned_consulted_every_step"] = {
+            "s_per_step": round(lrn_all["elapsed"] / args.steps, 6),
+            "iters_per_step": round(float(np.mean(lrn_all["iters"])), 2),
+            "outer_per_step": round(float(np.mean(lrn_all["outer"])), 2),
+            "warm_start_used_steps": lrn_all["warm_used"],
+            "consulted_steps": lrn_all["warm_consulted"],
+        }
         extra["without_learned_warm_start"] = {
             "s_per_step": round(s_plain, 6),
             "iters_per_step": round(float(np.mean(g["iters"])), 2),
