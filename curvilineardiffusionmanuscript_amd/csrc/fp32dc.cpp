@@ -122,7 +122,7 @@ int cdiff_solver::ensure_fp32() {
   lin = lin_ok();
   if (lin && ndim == 2) {
     // eight iterations per launch, marching up strips of 48 columns (4096^2:
-    // 42.4 us per iteration; 53.1 at depth 6, 75.5 at depth 4).  On
+    // 39.0 us per iteration; 54.9 at depth 6, 74.1 at depth 4).  On
     // wave-aligned 52 x 20 tiles six iterations take 79 us per iteration (100
     // at depth 4 on 64 x 16 tiles): tiles re-read their ring
     lin_k = tune.dc_depth2d;
@@ -135,17 +135,30 @@ int cdiff_solver::ensure_fp32() {
     lin_chunks = (nj + rows - 1) / rows;
     lin_tiles = ((ni + cols - 1) / cols) * lin_chunks;
     if (lin_layout == 2) {
-      // marching sweep: strips of 64 - 2 lin_k columns; chunks of rows sized
-      // for about 24 wavefronts per CU at depth 8 (two resident per SIMD:
-      // the grid runs in about three rounds; 4096^2: 64 rows per chunk, 42.4
-      // us per iteration against 46.3 at 48 rows, 48.6 at 80, 54.6 at 96) and
-      // 12 at depth 6 (three resident per SIMD; 128 rows, 53.1 us against
-      // 55.6 at 64), at least 2 lin_k rows each (a multiple of 8 chunks when
-      // there are that many: one set per XCD)
+      // marching sweep: strips of 64 - 2 lin_k columns, one wavefront per
+      // strip and chunk of rows.  A wavefront takes about the same time per
+      // row whether the GPU is full or not, and each marches 2 lin_k rows
+      // more than its chunk holds, so a launch lasts about
+      //   rounds x (rows per chunk + 2 lin_k)  row steps,
+      // rounds = the wavefronts over those resident at once; the chunk
+      // height is the one that minimises that, at least 2 lin_k rows (4096^2,
+      // depth 8, two wavefronts per SIMD: 23 chunks of 179 rows in one round,
+      // 195 row steps; chunks of 64 rows took three rounds, 240 row steps)
       march_strips = (ni + 63 - 2 * lin_k) / (64 - 2 * lin_k);
-      int want = std::max(1, (lin_k >= 8 ? 24 : 12) * ncu / march_strips);
-      if (want >= 8) want -= want % 8;
-      march_rows = std::max(2 * lin_k, (nj + want - 1) / want);
+      const int64_t slots = (int64_t)ncu * std::max(1, lin2d_march_resident_waves(lin_k));
+      int64_t best = -1;
+      march_rows = nj;
+      for (int c = 1; c <= std::max(1, nj / (2 * lin_k)); ++c) {
+        const int rws = (nj + c - 1) / c;
+        if (c > 1 && (nj + rws - 1) / rws != c) continue;  // the same chunks as a smaller c
+        const int64_t rounds = ((int64_t)c * march_strips + slots - 1) / slots;
+        const int64_t cost = rounds * (rws + 2 * lin_k);
+        if (best < 0 || cost < best) {
+          best = cost;
+          march_rows = rws;
+        }
+      }
+      march_rows = std::max(2 * lin_k, march_rows);
       if (tune.dc_march_rows > 0) march_rows = std::max(2 * lin_k, (int)tune.dc_march_rows);
       march_chunks = (nj + march_rows - 1) / march_rows;
     }

@@ -20,6 +20,12 @@ __device__ __forceinline__ float lin_cross(float xn, float xnw, float xs, float 
   return ((xn + xnw) - (xs + xsw)) + ((ye + yes) - (yw + yws));
 }
 
+// lin_cross from its four pair sums: sx_n = x_n + x_nw, sx_s = x_s + x_sw,
+// sy_e = y_e + y_es, sy_w = y_w + y_ws (bit for bit the same value)
+__device__ __forceinline__ float lin_cross_sums(float sx_n, float sx_s, float sy_e, float sy_w) {
+  return (sx_n - sx_s) + (sy_e - sy_w);
+}
+
 // D = m0 x_c + m1 x_w + m2 y_c + m3 y_s + m4 cross
 __device__ __forceinline__ float lin_div(float m0, float m1, float m2, float m3, float m4, float xc, float xw, float yc,
                                          float ys, float cross) {

@@ -35,17 +35,21 @@
 // Bound.  HBM: at 4096^2 (six iterations per launch, 53 us per iteration) a
 // copy without the arithmetic takes as long, one without the loads 27 us.  Its
 // access pattern -- each wavefront reads 256 bytes of 16 arrays per row, the
-// strips of a chunk in step, chunks apart -- moves ~4.1-4.3 TB/s in isolation
-// (a grid-stride read of the same arrays, 6.2 TB/s); neither more rows in
-// flight per wavefront nor interleaved coefficient records change that much.
-// A launch reads its 76 bytes per cell whatever its depth, so the default
-// runs eight iterations per launch (211 VGPRs, two wavefronts per SIMD; 42.4
-// us per iteration, the access pattern alone ~300 us per launch).
+// strips of a chunk in step, chunks apart -- moves ~4.1-4.5 TB/s (a
+// grid-stride read of the same arrays, 6.2 TB/s); neither more rows in flight
+// per wavefront nor interleaved coefficient records change that much, and
+// 12 % fewer VALU instructions gain 0.5 %.  A launch reads its 76 bytes per
+// cell whatever its depth, so the default runs eight iterations per launch
+// (226 VGPRs, two wavefronts per SIMD), and each chunk re-reads 2K rows, so
+// the chunks are as tall as one round of resident wavefronts allows (4096^2:
+// 23 chunks of 179 rows, 39 us per iteration; 45.5 with chunks of 64 rows in
+// three rounds).
 //
 // Values outside the dependency cone of the strip's cells (the ring's edge
 // lanes and the first rows of a chunk) are computed from incomplete
 // neighbours and never reach a written cell: after stage n, lanes
 // n .. 63 - n and rows from ya - K + n up are exact.
+#include <algorithm>
 #include <type_traits>
 
 #include "cdiff_dev.h"
@@ -99,17 +103,18 @@ sweep2d_lin_march_kernel(Lin2DArgs A) {
   if (A.ctrl->done) return;
   const Geo& g = A.g;
   const int lane = threadIdx.x & (WAVE - 1);
-  // XCD-aware split: block b runs on XCD b mod 8; the blocks of one XCD take
-  // the chunks c = XCD, XCD + 8, ... and, per chunk, groups of MWAVES
-  // neighbouring strips in order, so strips that share the cache lines at
-  // their edges share that XCD's L2 (they march up the same rows together)
+  // XCD-aware split: block b runs on XCD b mod 8 and the grid is a multiple
+  // of 8 blocks; the blocks of one XCD take one contiguous range of the
+  // (chunk, strip) items, chunk by chunk and strips in order, so strips that
+  // share the cache lines at their edges share that XCD's L2 (they march up
+  // the same rows together)
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int b = blockIdx.x, k = b >> 3;
-  const int groups = (A.strips + MWAVES - 1) / MWAVES;
-  const int chunk = (b & 7) + 8 * (k / groups);
-  const int strip = (k % groups) * MWAVES + w;
+  const int b = blockIdx.x;
+  const int item = ((b & 7) * (int)(gridDim.x >> 3) + (b >> 3)) * MWAVES + w;
+  const int chunk = item / A.strips;
+  const int strip = item - chunk * A.strips;
   const int part = b * MWAVES + w;
-  if (chunk >= A.chunks || strip >= A.strips) {
+  if (chunk >= A.chunks) {
     if (CHECK && lane == 0) {
       A.part_sum[part] = 0.0;
       A.part_max[part] = 0.0;
@@ -140,11 +145,13 @@ sweep2d_lin_march_kernel(Lin2DArgs A) {
   // sits in slot (r - t0) mod R
   float FX[R][6], CY[R][10];
   // E[n][0]: iteration-n correction of row t - n, E[n][1]: of row t - n - 1
-  // QX[n][0..2]: iteration-n x faces of rows t - n + 1, t - n, t - n - 1
+  // QX[n][0..1]: iteration-n x faces of rows t - n + 1, t - n
   // QY[n][0..1]: iteration-n y faces of rows t - n, t - n - 1
   // (level 0: the loaded values, rows t and t - 1)
-  float E[K + 1][2], QX[K + 1][3], QY[K + 1][2];
-  float QPX[3], QPY[2];  // iteration-K q' (check launches)
+  // SX[n][0..2]: x_n + x_nw of the x faces of rows t - n + 1 .. t - n - 1 (the
+  // cross term's sums, formed once per face row and used again two rows up)
+  float E[K + 1][2], QX[K + 1][2], QY[K + 1][2], SX[K + 1][3];
+  float QPX[2], QPY[2], SPX[3];  // iteration-K q' (check launches)
 #pragma unroll
   for (int s = 0; s < R; ++s) {
 #pragma unroll
@@ -155,10 +162,12 @@ sweep2d_lin_march_kernel(Lin2DArgs A) {
 #pragma unroll
   for (int n = 0; n <= K; ++n) {
     E[n][0] = E[n][1] = 0.f;
-    QX[n][0] = QX[n][1] = QX[n][2] = 0.f;
+    QX[n][0] = QX[n][1] = 0.f;
     QY[n][0] = QY[n][1] = 0.f;
+    SX[n][0] = SX[n][1] = SX[n][2] = 0.f;
   }
-  QPX[0] = QPX[1] = QPX[2] = QPY[0] = QPY[1] = 0.f;
+  QPX[0] = QPX[1] = QPY[0] = QPY[1] = 0.f;
+  SPX[0] = SPX[1] = SPX[2] = 0.f;
   double sq = 0.0, smax = 0.0;
 
   // x faces and the correction of row r (the correction masked to its valid
@@ -202,13 +211,15 @@ sweep2d_lin_march_kernel(Lin2DArgs A) {
 #pragma unroll
     for (int n = K; n >= 0; --n) {
       E[n][1] = E[n][0];
-      QX[n][2] = QX[n][1];
       QX[n][1] = QX[n][0];
       QY[n][1] = QY[n][0];
+      SX[n][2] = SX[n][1];
+      SX[n][1] = SX[n][0];
     }
-    QPX[2] = QPX[1];
     QPX[1] = QPX[0];
     QPY[1] = QPY[0];
+    SPX[2] = SPX[1];
+    SPX[1] = SPX[0];
     // ---- row t enters at level 0; the loads of the next rows are in flight
     //      while the stages run
     E[0][0] = FX[P][3];
@@ -240,10 +251,15 @@ sweep2d_lin_march_kernel(Lin2DArgs A) {
       }
       // cells of row j
       {
-        const float xc = QX[n][1], xn = QX[n][0], xs = QX[n][2];
+        const float xc = QX[n][1], xn = QX[n][0];
         const float yc = QY[n][0], ys = QY[n][1];
-        const float cross = lin_cross(xn, dpp_from_left(xn), xs, dpp_from_left(xs), dpp_from_right(yc),
-                                      dpp_from_right(ys), dpp_from_left(yc), dpp_from_left(ys));
+        // the sums of lin_cross formed lane-wise and shifted afterwards: the
+        // east pair y_e + y_es is the east lane's y_c + y_s, the west pair the
+        // west lane's, and x_s + x_sw is this stage's x_n + x_nw of two rows
+        // below -- the same operands in the same order, so the same bits
+        SX[n][0] = xn + dpp_from_left(xn);
+        const float sy = yc + ys;
+        const float cross = lin_cross_sums(SX[n][0], SX[n][2], dpp_from_right(sy), dpp_from_left(sy));
         const float D = lin_div(cy[5], cy[6], cy[7], cy[8], cy[9], xc, dpp_from_left(xc), yc, ys, cross);
         const float eo = E[n - 1][1];
         const float en = lin_update(eo, cy[4], cy[3], D, inv_dt);
@@ -256,10 +272,11 @@ sweep2d_lin_march_kernel(Lin2DArgs A) {
         if (CHECK && n == K) {
           // the residual of iteration K with its q' (all lanes shuffle; only
           // the strip's domain cells of the chunk's rows are summed)
-          const float pxc = QPX[1], pxn = QPX[0], pxs = QPX[2];
+          const float pxc = QPX[1], pxn = QPX[0];
           const float pyc = QPY[0], pys = QPY[1];
-          const float crossp = lin_cross(pxn, dpp_from_left(pxn), pxs, dpp_from_left(pxs), dpp_from_right(pyc),
-                                         dpp_from_right(pys), dpp_from_left(pyc), dpp_from_left(pys));
+          SPX[0] = pxn + dpp_from_left(pxn);
+          const float spy = pyc + pys;
+          const float crossp = lin_cross_sums(SPX[0], SPX[2], dpp_from_right(spy), dpp_from_left(spy));
           const float Dp = lin_div(cy[5], cy[6], cy[7], cy[8], cy[9], pxc, dpp_from_left(pxc), pyc, pys, crossp);
           const double res = (dom && cols && j >= ya && j < yb) ? (double)lin_residual(cy[3], en, inv_dt, Dp) : 0.0;
           sq += res * res;
@@ -297,7 +314,30 @@ sweep2d_lin_march_kernel(Lin2DArgs A) {
 }
 
 int lin2d_march_blocks(int strips, int chunks) {
-  return 8 * ((strips + MWAVES - 1) / MWAVES) * ((chunks + 7) / 8);
+  const int64_t blocks = ((int64_t)strips * chunks + MWAVES - 1) / MWAVES;
+  return (int)(8 * ((blocks + 7) / 8));
+}
+
+int lin2d_march_resident_waves(int kmax) {
+  // the fewer of the check and plain variants (the same register budget)
+  int a = 0, c = 0;
+  hipError_t e1, e2;
+  switch (kmax) {
+    case 4:
+      e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, sweep2d_lin_march_kernel<4, true, 1>, MWAVES * WAVE, 0);
+      e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, sweep2d_lin_march_kernel<4, false, 1>, MWAVES * WAVE, 0);
+      break;
+    case 8:
+      e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, sweep2d_lin_march_kernel<8, true, 1>, MWAVES * WAVE, 0);
+      e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, sweep2d_lin_march_kernel<8, false, 1>, MWAVES * WAVE, 0);
+      break;
+    default:
+      e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, sweep2d_lin_march_kernel<6, true, 1>, MWAVES * WAVE, 0);
+      e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, sweep2d_lin_march_kernel<6, false, 1>, MWAVES * WAVE, 0);
+      break;
+  }
+  if (e1 != hipSuccess || e2 != hipSuccess) return 0;
+  return MWAVES * std::min(a, c);
 }
 
 int lin2d_march_parts(int strips, int chunks) { return MWAVES * lin2d_march_blocks(strips, chunks); }
@@ -329,7 +369,8 @@ void launch_march_pf(const Lin2DArgs& A, int kmax, bool check, hipStream_t s, in
 void launch_sweep2d_march(const Lin2DArgs& A, int kmax, bool check, hipStream_t s) {
   // rows fetched ahead: 1 (4096^2, six iterations per launch: 53 us per
   // iteration; 2 and 3 rows ahead need 182 and 199 VGPRs, two wavefronts per
-  // SIMD, and take 60 us; eight iterations per launch, the default, 42.4 us)
+  // SIMD, and take 60 us; eight iterations per launch, the default: 39.0 us,
+  // two rows ahead 39.3 us at 247 VGPRs)
   launch_march_pf<1>(A, kmax, check, s, lin2d_march_blocks(A.strips, A.chunks));
 }
 

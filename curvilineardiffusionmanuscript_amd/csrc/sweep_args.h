@@ -142,6 +142,8 @@ void launch_sweep2d_lin(const Lin2DArgs& A, int kmax, int layout, bool check, in
 // partial sums: lin2d_march_parts(strips, chunks) entries
 int lin2d_march_blocks(int strips, int chunks);
 int lin2d_march_parts(int strips, int chunks);
+// wavefronts of the marching sweep resident on one CU at once (0 if unknown)
+int lin2d_march_resident_waves(int kmax);
 void launch_sweep2d_march(const Lin2DArgs& A, int kmax, bool check, hipStream_t s);
 
 // FP32 correction sweep in coefficient form, 3D (sweep3d_lin.hip): 64 x 8
