@@ -86,16 +86,16 @@ namespace {
 
 // Work in units of one FP32 correction sweep at the same mesh size, from
 // measurements at 4096^2 on MI355X (all three scale with the cell count): an
-// FP64 APT iteration ~9.2 sweeps (~390 vs 42.4 us), a fused outer pass of the
-// defect correction ~9.2 sweeps, and one consultation of the network
+// FP64 APT iteration ~10.5 sweeps (~385 vs 36.8 us), a fused outer pass of the
+// defect correction ~10 sweeps (365 us), and one consultation of the network
 // (the features with the FP64 residual of T^n, inference ~0.8 ms, the FP64
-// residual of the predicted start) ~50 sweeps (~2.1 ms)
-constexpr double WORK_FP64_ITER = 9.2, WORK_OUTER_PASS = 9.2, WORK_CONSULT = 50.0;
+// residual of the predicted start) ~62 sweeps (~2.27 ms)
+constexpr double WORK_FP64_ITER = 10.5, WORK_OUTER_PASS = 10.0, WORK_CONSULT = 62.0;
 
 }  // namespace
 
 // after a consultation that did not pay, the network sits out 15, 31, 63,
-// ... (at most 255) steps before it is consulted again: one trial costs ~50
+// ... (at most 255) steps before it is consulted again: one trial costs ~62
 // sweeps, a few percent of the 15 or more steps it then sits out
 void cdiff_solver::warm_back_off() {
   ++warm_rejects;
@@ -132,9 +132,9 @@ void cdiff_solver::warm_account(const cdiff_stats& st, bool dc) {
 // Whether a consultation is expected to pay: the last step's work per decade
 // times the decades a prediction is expected to take off (1 before the first
 // used prediction; then what the last one took off) against its cost.  On the
-// FP64 APT a decade costs ~100-250 sweep units at 4096^2 and a consultation
+// FP64 APT a decade costs ~110-270 sweep units at 4096^2 and a consultation
 // pays; the mixed-precision step needs ~13-30 units per decade, so a
-// prediction would have to take 1.7-4 decades off the start, more than the
+// prediction would have to take 2-5 decades off the start, more than the
 // BF16 network's ~1.
 bool cdiff_solver::warm_pays() const {
   const bool pays = work_per_decade * warm_decades > WORK_CONSULT;

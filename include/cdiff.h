@@ -306,8 +306,10 @@ int cdiff_reset_state(cdiff_solver *h);
 /* Measurement helper (no counterpart in the reference): run `nsweeps`
  * back-to-back fused sweeps of the current state on the handle's stream
  * (FP64, or the FP32 correction sweep with precision = CDIFF_PREC_FP32_DC),
- * with (check bit 0 set) or without the residual, bracketed by HIP events;
- * returns the mean time per pseudo-iteration in milliseconds.  With check bit
+ * with (check bit 0 set) or without the residual, bracketed by HIP events
+ * (after one untimed launch of the same kind, so that the launches are timed
+ * and not their enqueueing); returns the mean time per pseudo-iteration in
+ * milliseconds.  With check bit
  * 1 set, the iterations run two per launch (2D without periodic faces; even
  * nsweeps); with bit 2 and precision = CDIFF_PREC_FP32_DC, the coefficient-form
  * correction sweep runs at its full depth per launch (after an FP32 step;

@@ -131,8 +131,8 @@ def test_warm_started_step_reaches_the_same_tolerance(precision):
 
 def test_network_sits_out_where_it_cannot_pay():
     # the mixed-precision hot disk: a decade of residual reduction costs the
-    # step ~13-30 correction sweeps, so a prediction would have to take 1.7-4
-    # decades off the start to repay the ~50 sweeps a consultation costs;
+    # step ~13-30 correction sweeps, so a prediction would have to take 2-5
+    # decades off the start to repay the ~62 sweeps a consultation costs;
     # the network is never consulted and every step is the plain
     # mixed-precision step, bit for bit.  Consulted anyway (warm_policy 1),
     # the steps still end at the tolerance.
