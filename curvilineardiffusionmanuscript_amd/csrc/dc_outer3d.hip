@@ -20,8 +20,8 @@
 //            the face planes are double-buffered in LDS (plane p's faces in
 //            buffer p & 1), so the residual reads the faces of planes p-1 and
 //            p there and carries only the face sums of plane p-2.
-// The cell inputs of the next plane step are loaded one step ahead, the
-// residual inputs at the start of the step that uses them.
+// The cell inputs of the next plane step and the residual inputs of the
+// next residual are loaded one plane step ahead.
 #include <limits>
 
 #include "column3d.h"
@@ -298,10 +298,8 @@ __global__ __launch_bounds__(OT) void k_dc_outer3d(DcOuter3D A) {
   // residual of plane k (faces in buffer k & 1, those of plane k+1 and the
   // zeta faces stored at k in buffer (k+1) & 1, the zeta faces stored at k-1
   // in buffer k & 1) with the face sums xm, ym of plane k-1
-  auto residual = [&](int k, double xm, double ym, double un, double an) {
+  auto residual = [&](int k, const ResIn& R, double xm, double ym, double un, double an) {
     if (!col) return;
-    ResIn R{};
-    fetch_res(k, R);
     const int kb = k & 1, nb = (k + 1) & 1;
     const Plane pc = read_plane(kb);
     const double pn_xsum = xsum_of(nb), pn_ysum = ysum_of(nb);
@@ -350,17 +348,25 @@ __global__ __launch_bounds__(OT) void k_dc_outer3d(DcOuter3D A) {
   u_c = u_own;
   a_c = a_own;
   fetch_cells(z0 + 1, C);
+  // the residual inputs of a plane are loaded one plane step before the
+  // residual that uses them
+  ResIn Rn{};
+  fetch_res(z0, Rn);
   __syncthreads();
   faces(z0, true);
   __syncthreads();
   // ---- plane steps z0+1 .. z1+1: residual of plane p-1
   for (int p = z0 + 1; p <= z1 + 1; ++p) {
+    const ResIn Rc = Rn;
     cells(p, C);
-    if (p <= z1) fetch_cells(p + 1, C);
+    if (p <= z1) {
+      fetch_cells(p + 1, C);
+      fetch_res(p, Rn);
+    }
     __syncthreads();
     faces(p, true);
     __syncthreads();
-    residual(p - 1, xm, ym, u_c, a_c);
+    residual(p - 1, Rc, xm, ym, u_c, a_c);
     xm = xsum_of((p - 1) & 1);
     ym = ysum_of((p - 1) & 1);
     u_c = u_own;
