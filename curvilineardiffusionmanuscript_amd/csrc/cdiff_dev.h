@@ -6,6 +6,7 @@
 // a 128-byte boundary, and pitch a multiple of ALIGN.  Host arrays (Julia
 // column-major, unpadded) are converted by pitched copies.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -167,6 +168,20 @@ template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
   for (int o = WAVE / 2; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, WAVE));
   return v;
+}
+
+// A marching loop unrolled over a ring period R: step(integral_constant<P>,
+// t) for t = t, t + 1, ... with P = 0 .. R - 1, so that ring slots indexed by
+// P are compile-time constants; false once t passes t1.
+template <int P, int R, class F>
+__device__ __forceinline__ bool march_phases(F& step, int& t, int t1) {
+  if constexpr (P == R) {
+    return true;
+  } else {
+    step(std::integral_constant<int, P>{}, t);
+    if (++t > t1) return false;
+    return march_phases<P + 1, R>(step, t, t1);
+  }
 }
 
 }  // namespace cdiff

@@ -81,18 +81,6 @@ __device__ __forceinline__ void stf(float* __restrict__ base, uint32_t byte_offs
   *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset) = v;
 }
 
-// the phases 0 .. R - 1 of the unrolled row loop; false once the last row is done
-template <int P, int R, class F>
-__device__ __forceinline__ bool march_phases(F& step, int& t, int t1) {
-  if constexpr (P == R) {
-    return true;
-  } else {
-    step(std::integral_constant<int, P>{}, t);
-    if (++t > t1) return false;
-    return march_phases<P + 1, R>(step, t, t1);
-  }
-}
-
 }  // namespace
 
 template <int K, bool CHECK, int PF>

@@ -134,37 +134,46 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
   };
 
   // loads of the next planes: step t consumes the register set of plane t and
-  // refills it with plane t+PD.  Without the check, two sets (PD = 2, the loop
-  // unrolled by two to keep them in registers) put two planes' loads in flight
-  // behind the update; the check variant's residual needs the registers of the
-  // second set (PD = 1)
+  // refills it with plane t+PD.  Without the check, two sets (PD = 2,
+  // alternating with the step) put two planes' loads in flight behind the
+  // update; the check variant's residual needs the registers of the second
+  // set (PD = 1)
   constexpr int PD = CHECK ? 1 : 2;
-  Fetch FA, FB;
+  Fetch FS[PD];
   Upd U1{}, U2{}, Un{};
-  fetch(z0 - 2, FA, false);
-  if (PD == 2) fetch(z0 - 1, FB, true);
+  const int t0 = z0 - 2, t1 = z1 + 2;
+  fetch(t0, FS[0], false);
+  if (PD == 2) fetch(t0 + 1, FS[PD - 1], true);
   double sq = 0.0, smax = 0.0;
-  auto step = [&](int t, Fetch& F) {
+  // The LDS rings hold plane q in slot (q - t0) mod 2 (e, the zeta faces
+  // between q and q + 1, the face coefficients) or mod 3 (the xi and eta
+  // faces).  The plane loop is unrolled over the period 6, so with
+  // P = (t - t0) mod 6 every slot is a compile-time constant.
+  auto step = [&](auto Pc, int t) {
+    constexpr int P = decltype(Pc)::value;
+    // slots of plane t - d
+    constexpr auto s2 = [](int d) { return (P - d + 6) % 2; };
+    constexpr auto s3 = [](int d) { return (P - d + 6) % 3; };
+    Fetch& F = FS[P % PD];
     // ---- 1-2: e0(t) into LDS; iteration-1 faces of plane t (+ zeta faces t-1|t)
-    if (valid) sE0[t & 1][r] = F.e;
+    if (valid) sE0[s2(0)][r] = F.e;
     __syncthreads();
     const bool zf1 = t >= z0 - 1;
     if (valid) {
-      const float* E = sE0[t & 1];
-      const float* Em = sE0[(t + 1) & 1];
+      const float* E = sE0[s2(0)];
+      const float* Em = sE0[s2(1)];
       const float ec = E[r];
-      const int s = (t + 3) % 3;
       if (xf1) {
         const float px = F.f[0][0] * ec - F.f[0][1] * E[r + 1];
-        sX1[s][r] = px + F.f[0][2] * (F.f[0][3] - px);
+        sX1[s3(0)][r] = px + F.f[0][2] * (F.f[0][3] - px);
       }
       if (yf1) {
         const float py = F.f[1][0] * ec - F.f[1][1] * E[r + Q2X];
-        sY1[s][r] = py + F.f[1][2] * (F.f[1][3] - py);
+        sY1[s3(0)][r] = py + F.f[1][2] * (F.f[1][3] - py);
       }
       if (zf1) {
         const float pz = F.f[2][0] * Em[r] - F.f[2][1] * ec;
-        sZ1[(t + 1) & 1][r] = pz + F.f[2][2] * (F.f[2][3] - pz);
+        sZ1[s2(1)][r] = pz + F.f[2][2] * (F.f[2][3] - pz);
       }
     }
     // the face coefficients of plane t, for its iteration-2 faces one step
@@ -173,10 +182,10 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) sC[t & 1][3 * ax + q][r] = F.f[ax][q];
+        for (int q = 0; q < 3; ++q) sC[s2(0)][3 * ax + q][r] = F.f[ax][q];
     }
     // loads of the next plane step and of the update coefficients of plane t
-    if (t + PD <= z1 + 2) fetch(t + PD, F, t + PD >= z0 - 1);  // F is free now
+    if (t + PD <= t1) fetch(t + PD, F, t + PD >= z0 - 1);  // F is free now
     if (t >= z0 - 1 && t <= z1 + 1) fetch_upd(t, Un);
     __syncthreads();
 
@@ -186,38 +195,36 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
       if (inner) {
         float e1 = 0.f;
         if (in_dom(p1)) {
-          const int sm = (p1 + 2) % 3, sc = (p1 + 3) % 3, sp = (p1 + 4) % 3;
-          const int zc = (p1 + 2) & 1, zm = (p1 + 1) & 1;
-          const float D = div(U1, sX1[sm], sX1[sc], sX1[sp], sY1[sm], sY1[sc], sY1[sp], sZ1[zm], sZ1[zc]);
-          const float e = sE0[p1 & 1][r];
+          const float D = div(U1, sX1[s3(2)], sX1[s3(1)], sX1[s3(0)], sY1[s3(2)], sY1[s3(1)], sY1[s3(0)],
+                              sZ1[s2(2)], sZ1[s2(1)]);
+          const float e = sE0[s2(1)][r];
           e1 = e + U1.cc * (U1.rb - D - e * inv_dt);
         }
-        sE1[p1 & 1][r] = e1;
+        sE1[s2(1)][r] = e1;
       }
       __syncthreads();
 
       // ---- 4: iteration-2 faces of plane t-1 (+ zeta faces t-2|t-1)
       if (inner) {
-        const float(*Cf)[Q2] = sC[p1 & 1];
-        const float* E = sE1[p1 & 1];
-        const float* Em = sE1[(p1 + 1) & 1];
+        const float(*Cf)[Q2] = sC[s2(1)];
+        const float* E = sE1[s2(1)];
+        const float* Em = sE1[s2(2)];
         const float ec = E[r];
-        const int s = (p1 + 3) % 3;
         if (xf2) {
           const float px = Cf[0][r] * ec - Cf[1][r] * E[r + 1];
-          sX2[s][r] = px + Cf[2][r] * (sX1[s][r] - px);
-          if (CHECK) sPX[s][r] = px;
+          sX2[s3(1)][r] = px + Cf[2][r] * (sX1[s3(1)][r] - px);
+          if (CHECK) sPX[s3(1)][r] = px;
         }
         if (yf2) {
           const float py = Cf[3][r] * ec - Cf[4][r] * E[r + Q2X];
-          sY2[s][r] = py + Cf[5][r] * (sY1[s][r] - py);
-          if (CHECK) sPY[s][r] = py;
+          sY2[s3(1)][r] = py + Cf[5][r] * (sY1[s3(1)][r] - py);
+          if (CHECK) sPY[s3(1)][r] = py;
         }
         if (p1 >= z0) {
-          const int z = (p1 + 1) & 1;  // face p1-1 | p1
+          // face p1-1 | p1
           const float pz = Cf[6][r] * Em[r] - Cf[7][r] * ec;
-          sZ2[z][r] = pz + Cf[8][r] * (sZ1[z][r] - pz);
-          if (CHECK) sPZ[z][r] = pz;
+          sZ2[s2(2)][r] = pz + Cf[8][r] * (sZ1[s2(2)][r] - pz);
+          if (CHECK) sPZ[s2(2)][r] = pz;
         }
       }
       __syncthreads();
@@ -226,37 +233,32 @@ __global__ __launch_bounds__(T2N) void sweep3d_lin2_kernel(Lin3DArgs A) {
       const int p2 = t - 2;
       if (p2 >= z0 && owned) {
         const int64_t c = at(g, i, j, p2);
-        const int sm = (p2 + 2) % 3, sc = (p2 + 3) % 3, sp = (p2 + 4) % 3;
-        const int zc = (p2 + 2) & 1, zm = (p2 + 1) & 1;
-        const float D = div(U2, sX2[sm], sX2[sc], sX2[sp], sY2[sm], sY2[sc], sY2[sp], sZ2[zm], sZ2[zc]);
-        const float e = sE1[p2 & 1][r];
+        const float D = div(U2, sX2[s3(3)], sX2[s3(2)], sX2[s3(1)], sY2[s3(3)], sY2[s3(2)], sY2[s3(1)],
+                            sZ2[s2(3)], sZ2[s2(2)]);
+        const float e = sE1[s2(2)][r];
         const float en = e + U2.cc * (U2.rb - D - e * inv_dt);
         A.e_dst[c] = en;
-        A.q_dst[0][c] = sX2[sc][r];
-        A.q_dst[1][c] = sY2[sc][r];
-        A.q_dst[2][c] = sZ2[zc][r];
+        A.q_dst[0][c] = sX2[s3(2)][r];
+        A.q_dst[1][c] = sY2[s3(2)][r];
+        A.q_dst[2][c] = sZ2[s2(2)][r];
         if (CHECK) {
-          const float Dp = div(U2, sPX[sm], sPX[sc], sPX[sp], sPY[sm], sPY[sc], sPY[sp], sPZ[zm], sPZ[zc]);
+          const float Dp = div(U2, sPX[s3(3)], sPX[s3(2)], sPX[s3(1)], sPY[s3(3)], sPY[s3(2)], sPY[s3(1)],
+                               sPZ[s2(3)], sPZ[s2(2)]);
           const double res = (double)(U2.rb - en * inv_dt - Dp);
           sq += res * res;
           smax = nanmax(smax, fabs(res));
         }
         // boundary faces below the domain that no cell owns
-        if (i == g.lo[0]) A.q_dst[0][c - 1] = sX2[sc][r - 1];
-        if (j == g.lo[1]) A.q_dst[1][c - g.pitch] = sY2[sc][r - Q2X];
-        if (p2 == g.lo[2]) A.q_dst[2][c - g.plane] = sZ2[zm][r];
+        if (i == g.lo[0]) A.q_dst[0][c - 1] = sX2[s3(2)][r - 1];
+        if (j == g.lo[1]) A.q_dst[1][c - g.pitch] = sY2[s3(2)][r - Q2X];
+        if (p2 == g.lo[2]) A.q_dst[2][c - g.plane] = sZ2[s2(3)][r];
       }
     }
     U2 = U1;
     U1 = Un;
   };
-  if constexpr (PD == 1) {
-    for (int t = z0 - 2; t <= z1 + 2; ++t) step(t, FA);
-  } else {
-    for (int t = z0 - 2; t <= z1 + 2; t += 2) {
-      step(t, FA);
-      if (t + 1 <= z1 + 2) step(t + 1, FB);
-    }
+  int t = t0;
+  while (march_phases<0, 6>(step, t, t1)) {
   }
   if (CHECK) {
     sq = wave_sum(sq);
