@@ -160,7 +160,7 @@ KERNELS = {
     ("fp64", 2): ("sweep2d_pair_kernel<double, false, 8>", 2, 0 | 2),
     ("fp32_dc", 2): ("sweep2d_lin_march_kernel<8, true, 1>", 8, 4 | 1),
     ("fp64", 3): ("sweep3d_col_kernel<double, false, true>", 1, 0),
-    ("fp32_dc", 3): ("sweep3d_lin2_kernel<false>", 2, 4),
+    ("fp32_dc", 3): ("sweep3d_lin2_kernel<true>", 2, 4 | 1),
 }
 
 
