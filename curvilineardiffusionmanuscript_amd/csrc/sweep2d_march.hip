@@ -158,18 +158,17 @@ sweep2d_lin_march_kernel(Lin2DArgs A) {
   SPX[0] = SPX[1] = SPX[2] = 0.f;
   double sq = 0.0, smax = 0.0;
 
-  // x faces and the correction of row r (the correction masked to its valid
-  // entries; on the first launch of a solve it is 0)
+  // x faces and the correction of row r, as loaded (the correction and its
+  // fluxes are masked when the row enters the pipeline, one step later: a
+  // select right after the load would wait for it there)
   auto load_x = [&](int r, float* fx) {
     const uint32_t o = offset(r);
     fx[0] = ldf(A.fx[0], o);
     fx[1] = ldf(A.fx[1], o);
     fx[2] = ldf(A.fx[2], o);
-    const float e = ldf(A.e_src, o), qx = ldf(A.qx_src, o), qy = ldf(A.qy_src, o);
-    const bool rin = !zero_src && r >= jlo && r <= jhi;
-    fx[3] = (icol && rin) ? e : 0.f;
-    fx[4] = (xcol && rin) ? qx : 0.f;
-    fx[5] = (icol && !zero_src && r >= jlo - 1 && r <= jhi) ? qy : 0.f;
+    fx[3] = ldf(A.e_src, o);
+    fx[4] = ldf(A.qx_src, o);
+    fx[5] = ldf(A.qy_src, o);
   };
   // y faces and cell coefficients of row r
   auto load_y = [&](int r, float* cy) {
@@ -208,13 +207,17 @@ sweep2d_lin_march_kernel(Lin2DArgs A) {
     QPY[1] = QPY[0];
     SPX[2] = SPX[1];
     SPX[1] = SPX[0];
-    // ---- row t enters at level 0; the loads of the next rows are in flight
-    //      while the stages run
-    E[0][0] = FX[P][3];
-    QX[0][0] = FX[P][4];
-    QY[0][0] = FX[P][5];
+    // ---- the loads of the next rows, in flight while the stages run
     if (t + PF - 1 < t1) load_y(t + PF - 1, CY[(P + PF - 1) % R]);
     if (t + PF <= t1) load_x(t + PF, FX[(P + PF) % R]);
+    // ---- row t enters at level 0: its correction and fluxes masked to their
+    //      valid entries (on the first launch of a solve they are 0)
+    {
+      const bool rin = !zero_src && t >= jlo && t <= jhi;
+      E[0][0] = (icol && rin) ? FX[P][3] : 0.f;
+      QX[0][0] = (xcol && rin) ? FX[P][4] : 0.f;
+      QY[0][0] = (icol && !zero_src && t >= jlo - 1 && t <= jhi) ? FX[P][5] : 0.f;
+    }
 
     // ---- stages
 #pragma unroll
