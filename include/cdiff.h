@@ -171,7 +171,8 @@ typedef struct {
   int32_t dc_fused3d;       /* 1: fused 3D outer pass (default), 0: separate kernels */
   int32_t dc_depth2d;       /* iterations per 2D correction launch: 2, 4, 6 or 8 (default) */
   int32_t dc_layout2d;      /* 0: 64-column tiles, 1: wave-aligned tiles, 2: marching strips (default) */
-  int32_t dc_march_rows;    /* rows per marching chunk, 0 = auto */
+  int32_t dc_march_rows;    /* rows per marching chunk, 0 = auto (the height that minimises rounds of resident
+                               wavefronts x (rows + 2 depth); 4096^2: 179, all wavefronts in one round) */
   int32_t dc_launch_iters;  /* iterations per launch, 1 .. depth; 0 = the depth */
   int32_t dc_depth3d;       /* iterations per 3D correction launch: 1 or 2 (default) */
   int32_t dc_planes3d;      /* planes per chunk of the one-iteration 3D sweep, 0 = auto */
