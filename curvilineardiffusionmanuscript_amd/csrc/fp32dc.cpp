@@ -136,14 +136,15 @@ int cdiff_solver::ensure_fp32() {
     lin_tiles = ((ni + cols - 1) / cols) * lin_chunks;
     if (lin_layout == 2) {
       // marching sweep: strips of 64 - 2 lin_k columns, one wavefront per
-      // strip and chunk of rows.  A wavefront takes about the same time per
-      // row whether the GPU is full or not, and each marches 2 lin_k rows
-      // more than its chunk holds, so a launch lasts about
+      // strip and chunk of rows; each wavefront marches 2 lin_k rows more
+      // than its chunk holds.  The launch time follows
       //   rounds x (rows per chunk + 2 lin_k)  row steps,
-      // rounds = the wavefronts over those resident at once; the chunk
-      // height is the one that minimises that, at least 2 lin_k rows (4096^2,
-      // depth 8, two wavefronts per SIMD: 23 chunks of 179 rows in one round,
-      // 195 row steps; chunks of 64 rows took three rounds, 240 row steps)
+      // rounds = the wavefronts over those resident at once (measured at
+      // 4096^2 for chunks of 32 to 96 rows: a partly filled last round costs
+      // nearly a full one), so the chunk height is the one that minimises it,
+      // at least 2 lin_k rows (4096^2, depth 8, two wavefronts per SIMD: 23
+      // chunks of 179 rows in one round, 195 row steps, 39.0 us per
+      // iteration; chunks of 64 rows took three rounds, 240 row steps, 45.5)
       march_strips = (ni + 63 - 2 * lin_k) / (64 - 2 * lin_k);
       const int64_t slots = (int64_t)ncu * std::max(1, lin2d_march_resident_waves(lin_k));
       int64_t best = -1;
