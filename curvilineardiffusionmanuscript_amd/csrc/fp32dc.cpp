@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -129,6 +130,8 @@ int cdiff_solver::ensure_fp32() {
     const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
     lin_layout = tune.dc_layout2d;
     if (lin_layout == 2 && lin_k == 2) lin_layout = 1;  // the marching sweep runs 4, 6 or 8 iterations
+    // the marching sweep addresses its arrays with 32-bit byte offsets
+    if (lin_layout == 2 && (int64_t)g.total * (int64_t)sizeof(float) > (int64_t)UINT32_MAX) lin_layout = 1;
     int cols = 64, rows = 16;
     // (launches of fewer than lin_k iterations use the wave-aligned tiles)
     lin2d_tile(lin_k, lin_layout == 0 ? 0 : 1, &cols, &rows);
