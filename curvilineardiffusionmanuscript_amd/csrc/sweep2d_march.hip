@@ -50,6 +50,7 @@
 // neighbours and never reach a written cell: after stage n, lanes
 // n .. 63 - n and rows from ya - K + n up are exact.
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "cdiff_dev.h"
@@ -310,7 +311,11 @@ int lin2d_march_blocks(int strips, int chunks) {
 }
 
 int lin2d_march_resident_waves(int kmax) {
-  // the fewer of the check and plain variants (the same register budget)
+  // the fewer of the check and plain variants (the same register budget);
+  // queried once per depth (the kernels' resources do not change)
+  static std::atomic<int> cached[3] = {0, 0, 0};
+  const int slot = kmax == 4 ? 0 : (kmax == 8 ? 1 : 2);
+  if (const int v = cached[slot].load(std::memory_order_relaxed)) return v;
   int a = 0, c = 0;
   hipError_t e1, e2;
   switch (kmax) {
@@ -328,7 +333,9 @@ int lin2d_march_resident_waves(int kmax) {
       break;
   }
   if (e1 != hipSuccess || e2 != hipSuccess) return 0;
-  return MWAVES * std::min(a, c);
+  const int v = MWAVES * std::min(a, c);
+  cached[slot].store(v, std::memory_order_relaxed);
+  return v;
 }
 
 int lin2d_march_parts(int strips, int chunks) { return MWAVES * lin2d_march_blocks(strips, chunks); }
