@@ -53,6 +53,7 @@ struct Ctrl {
   int grow;         // inner correction loop: consecutive checks whose residual grew
   int ref_iter;     // inner correction loop: iteration at which the residual last halved
   double ref;       //   and the residual it halved to
+  int final_buf;    // inner correction loop: buffer written by the launch of its last check
 };
 
 // v[f] of a 6-entry table (the per-face BC rules of a Ghost) for a face index

@@ -19,7 +19,7 @@ __global__ void k_dc_residual(Geo g, int harmonic, double dt, const double* u, c
                               double* psum, double* pmax);
 __global__ void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* psum, const double* pmax, int nparts,
                                  double nfull, int norm_kind, double eta, int iter, int cap, double floor_frac,
-                                 double rtol, double atol);
+                                 double rtol, double atol, int out_buf);
 __global__ void k_fill32(int64_t n, float v, float* dst);
 __global__ void k_ctrl_set_init(Ctrl* ctrl, const Ctrl* src, double rtol, double atol);
 __global__ void k_lin_metrics(Geo g, const double* g11, const double* g22, const double* c12, const double* ac,
@@ -49,6 +49,11 @@ struct DcOuter2D {
   const double* rho;
   const double* u_old;
   const float* e;  // nullable: no correction
+  // when e_sel is set, the correction is e_buf0 or e_buf1 as e_sel->final_buf
+  // says (the buffer the correction solve's last check saw written)
+  const Ctrl* e_sel;
+  const float* e_buf0;
+  const float* e_buf1;
   double* u_new;
   const double* uprev;
   const double* c[5];  // g11 g22 c12 ac bc
@@ -76,6 +81,11 @@ struct DcOuter3D {
   const double* rho;
   const double* u_old;
   const float* e;  // nullable: no correction
+  // when e_sel is set, the correction is e_buf0 or e_buf1 as e_sel->final_buf
+  // says (the buffer the correction solve's last check saw written)
+  const Ctrl* e_sel;
+  const float* e_buf0;
+  const float* e_buf1;
   double* u_new;
   const double* uprev;
   const double* c;  // g11 g22 g33 g12 g13 g23 ac bc gc: array q at c + q * cstride

@@ -197,7 +197,8 @@ __global__ __launch_bounds__(256) void k_dc_residual(Geo g, int harmonic, double
 __global__ __launch_bounds__(1024) void k_inner_finalize(Ctrl* cin, const Ctrl* cout, const double* __restrict__ psum,
                                                          const double* __restrict__ pmax, int nparts, double nfull,
                                                          int norm_kind, double eta, int iter, int cap,
-                                                         double floor_frac, double rtol, double atol) {
+                                                         double floor_frac, double rtol, double atol,
+                                                         int out_buf) {
   if (cin->done) return;
   constexpr int NT = 1024, U = 8;
   __shared__ double ss[NT], sm[NT];
@@ -235,6 +236,7 @@ __global__ __launch_bounds__(1024) void k_inner_finalize(Ctrl* cin, const Ctrl* 
   const double prev = cin->last_l2;
   cin->last_l2 = L2;
   cin->nchecks += 1;
+  cin->final_buf = out_buf;
   const double prev_best = cin->init;  // +inf after k_ctrl_reset
   const double best = fmin(prev_best, L2);
   cin->init = best;

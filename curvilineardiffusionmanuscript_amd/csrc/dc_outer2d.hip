@@ -81,9 +81,10 @@ __global__ __launch_bounds__(ONT) __attribute__((amdgpu_waves_per_eu(8))) void k
     um = A.uprev[cm];
   }
 
+  const float* Ae = A.e_sel ? ((A.e_sel->final_buf & 1) ? A.e_buf1 : A.e_buf0) : A.e;
   auto new_u = [&](int i, int r) {
     const int64_t c = at(g, i, r, 0);
-    return A.u_old[c] + (A.e ? (double)A.e[c] : 0.0);
+    return A.u_old[c] + (Ae ? (double)Ae[c] : 0.0);
   };
 
   // ---- cells of the ringed tile

@@ -78,6 +78,7 @@ __global__ __launch_bounds__(OT) void k_dc_outer3d(DcOuter3D A) {
   const double dt = A.ic.dt;
   const IterConst32 k32 = iter_const32(A.ic, A.dtau_scale, A.theta_scale);
   const float t_exp = (float)A.t_exp;
+  const float* Ae = A.e_sel ? ((A.e_sel->final_buf & 1) ? A.e_buf1 : A.e_buf0) : A.e;
 
   int rlx[2], rly[2];
   bool rv[2];
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(OT) void k_dc_outer3d(DcOuter3D A) {
       if (nout == 0) {
         C[s].kind = 1;
         C[s].v = A.u_old[c];
-        if (A.e) C[s].e = A.e[c];
+        if (Ae) C[s].e = Ae[c];
       } else if (nout == 1) {
         const int f = oi ? (i < g.lo[0] ? 0 : 1) : (oj ? (j < g.lo[1] ? 2 : 3) : (p < g.lo[2] ? 4 : 5));
         const int mode = pick6(A.gh.mode, f);
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(OT) void k_dc_outer3d(DcOuter3D A) {
           const int64_t sc = at(g, oi ? src : i, oj ? src : j, ok ? src : p);
           C[s].kind = mode == G_NEG ? -1 : 1;
           C[s].v = A.u_old[sc];
-          if (A.e) C[s].e = A.e[sc];
+          if (Ae) C[s].e = Ae[sc];
         }
       } else {
         C[s].kind = 3;

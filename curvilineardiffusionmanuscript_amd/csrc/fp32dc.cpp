@@ -437,7 +437,7 @@ bool cdiff_solver::lin_ok() const {
 }
 
 int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, double* u_new, const double* uprev,
-                                int iter, const cdiff_opts& o, bool finalize) {
+                                int iter, const cdiff_opts& o, bool finalize, const Ctrl* e_sel) {
   // (the fused passes keep alpha in LDS; the step's epilogue forms the final one)
   const int ni = g.hi[0] - g.lo[0] + 1, nj = g.hi[1] - g.lo[1] + 1;
   if (ndim == 3) {
@@ -465,6 +465,9 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
     B.rho = rho;
     B.u_old = u_old;
     B.e = e;
+    B.e_sel = e_sel;
+    B.e_buf0 = u32[0];
+    B.e_buf1 = u32[1];
     B.u_new = u_new;
     B.uprev = uprev;
     B.c = coef[0];
@@ -505,6 +508,9 @@ int cdiff_solver::dc_outer_pass(double dt, const double* u_old, const float* e, 
   A.rho = rho;
   A.u_old = u_old;
   A.e = e;
+  A.e_sel = e_sel;
+  A.e_buf0 = u32[0];
+  A.e_buf1 = u32[1];
   A.u_new = u_new;
   A.uprev = uprev;
   for (int c = 0; c < 5; ++c) A.c[c] = coef[c];
@@ -721,7 +727,7 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
           hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(1024), 0, stream, ctrl_in, (const Ctrl*)ctrl,
                              (const double*)(lin ? lin_psum : part_sum), (const double*)(lin ? lin_pmax : part_max),
                              lin ? lin_last_parts : nparts, N_full, o.norm_kind, fixed_inner > 0 ? 0.0 : eta, tl, cap,
-                             fixed_inner > 0 ? 0.0 : floor_frac, o.rtol, o.atol);
+                             fixed_inner > 0 ? 0.0 : floor_frac, o.rtol, o.atol, ebuf);
           LAUNCH_CHECK("k_inner_finalize");
         }
         t = tl + 1;
@@ -740,6 +746,17 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
       pending = true;
       ++seg;
     }
+    // The fused outer passes take the correction from the buffer that the
+    // solve's last check recorded (k_inner_finalize), so the pass is enqueued
+    // right behind the correction solve, and the control blocks of both are
+    // read together: the GPU does not wait for the host between them.  A
+    // solve that diverged is dropped below, and the pass redone without it.
+    const bool fused = lin && (ndim == 2 || tune.dc_fused3d);
+    const Ctrl outer_before = ctrl_h[2];  // the outer control block the pass starts from
+    if (fused) {
+      if ((rc = dc_outer_pass(dt, u[a], nullptr, u[1 - a], u_prev, outer + 2, o, true, ctrl_in))) return rc;
+      HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+    }
     HIP_TRY(hipMemcpyAsync(&ctrl_h[3], ctrl_in, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     const int ni = ctrl_h[3].final_iter > 0 ? ctrl_h[3].final_iter : it;
@@ -755,21 +772,27 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
       dc_Vpdtau = Vpdtau;
       dc_dtau_scale = dc_theta_scale = 1.0;
       dc_dtau_scale_dflt = dc_theta_scale_dflt = 1.0;
+      if (fused) {
+        // the pass above applied the dropped correction: its control block back
+        ctrl_h[0] = outer_before;
+        HIP_TRY(hipMemcpyAsync(ctrl, &ctrl_h[0], sizeof(Ctrl), hipMemcpyHostToDevice, stream));
+      }
       if ((rc = dc_residual(dt, a, nullptr, 1 - a, outer + 2, o))) return rc;
       a = 1 - a;
       HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
       HIP_TRY(hipStreamSynchronize(stream));
       continue;
     }
-    int eb = ends.empty() ? 0 : ends.back().second;
-    for (const auto& pr : ends)
-      if (pr.first == ni) eb = pr.second;
-    const float* e = u32[eb];
     ++outer;
-    if ((rc = dc_residual(dt, a, e, 1 - a, outer + 1, o))) return rc;
+    if (!fused) {
+      int eb = ends.empty() ? 0 : ends.back().second;
+      for (const auto& pr : ends)
+        if (pr.first == ni) eb = pr.second;
+      if ((rc = dc_residual(dt, a, u32[eb], 1 - a, outer + 1, o))) return rc;
+      HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+    }
     a = 1 - a;
-    HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
     if (dc_debug)
       std::fprintf(stderr, "[dc] outer %d: inner %d (inner residual %.3e), outer residual %.3e\n", outer, ni,
                    ctrl_h[3].last_l2, ctrl_h[2].abs_);

@@ -192,6 +192,7 @@ struct cdiff_solver {
   int dc_residual(double dt, int srcbuf, const float* e, int dstbuf, int iter, const cdiff_opts& o);
   int dc_finalize_outer(int iter, const cdiff_opts& o);  // norm of the last outer pass's partial sums
   // the fused outer pass on explicit buffers (2D, lin_ok); finalize: update ctrl
+  // e_sel: take the correction from u32[e_sel->final_buf] instead of e
   int dc_outer_pass(double dt, const double* u_old, const float* e, double* u_new, const double* uprev, int iter,
-                    const cdiff_opts& o, bool finalize);
+                    const cdiff_opts& o, bool finalize, const cdiff::Ctrl* e_sel = nullptr);
 };
