@@ -93,3 +93,45 @@ def test_device_mms_matches_oracle_and_order():
         errs.append(float(np.max(np.abs(Td - ue)[1:-1, 1:-1])))
         dev.close()
     assert math.log2(errs[1] / errs[2]) > 1.8, errs
+
+
+@pytest.mark.gpu
+def test_3d_mms_mixed_precision_matches_fp64_and_oracle():
+    # the forcing A + B alpha on the 3D mixed-precision path (the fused 3D
+    # outer pass adds it to the FP64 residual): on a 3D wavy grid the steady
+    # Gaussian of the MMS driver, three steps; each mixed-precision step ends
+    # at the FP64 tolerance, and its temperature agrees with the FP64 APT's
+    # (device and oracle, bit-identical to each other) to dt sqrt(N) atol
+    from curvilineardiffusionmanuscript_amd import solver as S
+    from curvilineardiffusionmanuscript_amd.mesh import CurvilinearGrid3D, wavy_map_3d
+
+    mesh = CurvilinearGrid3D.from_mapping(wavy_map_3d(0.03), (24, 20, 18), 1)
+    center, fwhm = (0.5, 0.5, 0.5), 0.05
+    T = tz.gaussian_exact(mesh, 0.0, center, fwhm, no_t=True)
+    A, B = tz.gaussian_forcing(mesh, 0.0, center, LAW, fwhm=fwhm, no_t=True)
+    bcs = [S.NeumannBC()] * 6
+    dt, atol = 2e-3, 1e-9
+    out = {}
+    for prec in ("fp32_dc", "fp64"):
+        dev = S.PseudoTransientSolver(mesh, bcs)
+        dev.set_forcing(A, B)
+        Td = T.copy(order="F")
+        S.update_conductivity(dev, mesh, Td, 1.0, 1.0, LAW)
+        for _ in range(3):
+            st, _ = S.nonlinear_thermal_conduction_step(dev, mesh, Td, 1.0, 1.0, LAW, dt, rtol=0.0, atol=atol,
+                                                        max_iter=10**6, calculate_next_dt=False,
+                                                        show_convergence=False, precision=prec)
+            assert st.abs_err <= atol
+            if prec == "fp32_dc":
+                assert st.fp64_fallback == 0
+        out[prec] = Td
+        dev.close()
+    ora = O.OracleSolver(mesh, [("neumann", 0.0)] * 6, kappa=(1.0, 2.5, 0.0))
+    ora.set_forcing(A, B)
+    To = T.copy(order="F")
+    ora.update_conductivity(To)
+    for _ in range(3):
+        ora.step(To, dt, rtol=0.0, atol=atol, max_iter=10**6, calculate_next_dt=False)
+    assert np.max(np.abs(out["fp64"] - To)) <= 1e-12 * np.max(np.abs(To))
+    bound = 3 * dt * math.sqrt(T.size) * atol
+    assert np.max(np.abs(out["fp32_dc"] - To)) <= bound
