@@ -327,7 +327,7 @@ int cdiff_create(int ndim, const int64_t* dims_full, int nhalo, int mean, const 
   HIP_TRY(hipMalloc((void**)&h->red, sizeof(double) * (8 + STEP_BLOCKS)));
   h->owned.push_back(h->red);
   HIP_TRY(hipHostMalloc((void**)&h->red_h, sizeof(double) * 8, hipHostMallocDefault));
-  for (int e = 0; e < 2; ++e) HIP_TRY(hipEventCreateWithFlags(&h->ev[e], hipEventDisableTiming));
+  for (int e = 0; e < 3; ++e) HIP_TRY(hipEventCreateWithFlags(&h->ev[e], hipEventDisableTiming));
   h->configure_sweep();
   if ((rc = h->ensure_parts())) return rc;
   HIP_TRY(hipMalloc((void**)&h->ic_dev, sizeof(IterConst)));
@@ -349,7 +349,7 @@ int cdiff_destroy(cdiff_solver* h) {
   for (void* p : h->owned32) (void)hipFree(p);
   if (h->ctrl_h) (void)hipHostFree(h->ctrl_h);
   if (h->red_h) (void)hipHostFree(h->red_h);
-  for (int e = 0; e < 2; ++e)
+  for (int e = 0; e < 3; ++e)
     if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

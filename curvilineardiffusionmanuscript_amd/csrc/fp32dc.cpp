@@ -671,95 +671,134 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     return c.status == CDIFF_NONFINITE || !(c.abs_ == c.abs_) || c.abs_ > 100.0 * r_first;
   };
   if (failed_now()) dc_failed = true;
-  while (!ctrl_h[2].done && !dc_failed) {
-    if (outer >= max_outer) {
-      dc_failed = true;
-      break;
-    }
-    // inner FP32 solve for the correction, from e = 0 and q = 0 (the
-    // coefficient-form sweeps read no state on their first launch)
-    if (!lin) {
-      hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, u32[0]);
-      for (int ax = 0; ax < ndim; ++ax)
-        for (int b = 0; b < 2; ++b)
-          hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, q32[ax][b]);
-    }
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl_in, 0.0, 0.0);
-    LAUNCH_CHECK("inner reset");
-    // (the coefficient-form sweep's coefficients come with the residual pass)
-    // the FP32 sweeps of a step never exceed max_iter: each correction solve
-    // gets what is left of it, and none is left once it is used up
-    const int64_t left = total_cap - inner_total;
-    if (left <= 0) {
-      dc_failed = true;
-      break;
-    }
-    const int cap = (int)std::min<int64_t>(fixed_inner > 0 ? fixed_inner : INT_MAX - 1, left);
-    int it = 0, seg = 0;
+  // A correction solve in flight: its iterations enqueued so far, the
+  // segment the host reads next, and the buffer each launch wrote.
+  struct Solve {
+    int it = 0, seg = 0, ebuf = 0, cap = 0;
     bool pending = false;
-    int ebuf = 0;  // source buffer of the next inner launch
     std::vector<std::pair<int, int>> ends;  // (last iteration, buffer written)
-    // the first segment runs to the check at or after the previous
-    // correction's sweep count, later ones one check each; one segment is
-    // always queued ahead of the host's read, so a correction that converges
-    // leaves at most one check's launches (which return at once) behind it
-    const int first_len = (int)std::min<int64_t>(
-        INT_MAX / 2, std::max<int64_t>(ichk, (std::max<int64_t>(dc_inner_guess, 1) + ichk - 1) / ichk * ichk));
-    while (true) {
-      const int s = it + 1;
-      const int e = (int)std::min<int64_t>(cap, (int64_t)s + (seg == 0 ? first_len : ichk) - 1);
-      for (int t = s; t <= e;) {
-        auto is_chk = [&](int x) { return fixed_inner > 0 ? (x == cap) : (x % ichk == 0 || x == cap); };
-        int tl = t;
-        if (lin) {
-          // up to lin_nit iterations; a check iteration ends the launch
-          while (tl < e && tl - t + 1 < lin_nit && !is_chk(tl)) ++tl;
-          if ((rc = launch_lin(dt, ebuf, is_chk(tl), tl - t + 1, t == 1))) return rc;
-        } else {
-          const bool pair = pairs && !is_chk(t) && t + 1 <= e;
-          tl = pair ? t + 1 : t;
-          if ((rc = launch_sweep32(dt, Vpdtau, ebuf, is_chk(tl), pair))) return rc;
-        }
-        const bool chk_last = is_chk(tl);
-        ebuf ^= 1;
-        ends.push_back({tl, ebuf});
-        if (chk_last) {
-          hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(1024), 0, stream, ctrl_in, (const Ctrl*)ctrl,
-                             (const double*)(lin ? lin_psum : part_sum), (const double*)(lin ? lin_pmax : part_max),
-                             lin ? lin_last_parts : nparts, N_full, o.norm_kind, fixed_inner > 0 ? 0.0 : eta, tl, cap,
-                             fixed_inner > 0 ? 0.0 : floor_frac, o.rtol, o.atol, ebuf);
-          LAUNCH_CHECK("k_inner_finalize");
-        }
-        t = tl + 1;
+  };
+  // enqueue iterations S.it + 1 .. S.it + len (at most S.cap) and the read of
+  // the inner control block after them (ctrl_h[S.seg & 1], event ev[S.seg & 1])
+  auto enqueue = [&](Solve& S, int len) -> int {
+    const int cap = S.cap;
+    auto is_chk = [&](int x) { return fixed_inner > 0 ? (x == cap) : (x % ichk == 0 || x == cap); };
+    const int s = S.it + 1;
+    const int e = (int)std::min<int64_t>(cap, (int64_t)s + len - 1);
+    for (int t = s; t <= e;) {
+      int tl = t;
+      if (lin) {
+        // up to lin_nit iterations; a check iteration ends the launch
+        while (tl < e && tl - t + 1 < lin_nit && !is_chk(tl)) ++tl;
+        if ((rc = launch_lin(dt, S.ebuf, is_chk(tl), tl - t + 1, t == 1))) return rc;
+      } else {
+        const bool pair = pairs && !is_chk(t) && t + 1 <= e;
+        tl = pair ? t + 1 : t;
+        if ((rc = launch_sweep32(dt, Vpdtau, S.ebuf, is_chk(tl), pair))) return rc;
       }
-      it = e;
-      HIP_TRY(hipMemcpyAsync(&ctrl_h[seg & 1], ctrl_in, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
-      HIP_TRY(hipEventRecord(ev[seg & 1], stream));
-      if (pending) {
-        HIP_TRY(hipEventSynchronize(ev[(seg - 1) & 1]));
-        if (dc_debug)
-          std::fprintf(stderr, "[dc]   inner check %d: residual %.3e\n", ctrl_h[(seg - 1) & 1].nchecks,
-                       ctrl_h[(seg - 1) & 1].last_l2);
-        if (ctrl_h[(seg - 1) & 1].done) break;
+      const bool chk_last = is_chk(tl);
+      S.ebuf ^= 1;
+      S.ends.push_back({tl, S.ebuf});
+      if (chk_last) {
+        hipLaunchKernelGGL(k_inner_finalize, dim3(1), dim3(1024), 0, stream, ctrl_in, (const Ctrl*)ctrl,
+                           (const double*)(lin ? lin_psum : part_sum), (const double*)(lin ? lin_pmax : part_max),
+                           lin ? lin_last_parts : nparts, N_full, o.norm_kind, fixed_inner > 0 ? 0.0 : eta, tl, cap,
+                           fixed_inner > 0 ? 0.0 : floor_frac, o.rtol, o.atol, S.ebuf);
+        LAUNCH_CHECK("k_inner_finalize");
       }
-      if (it >= cap) break;
-      pending = true;
-      ++seg;
+      t = tl + 1;
     }
-    // The fused outer passes take the correction from the buffer that the
-    // solve's last check recorded (k_inner_finalize), so the pass is enqueued
-    // right behind the correction solve, and the control blocks of both are
-    // read together: the GPU does not wait for the host between them.  A
-    // solve that diverged is dropped below, and the pass redone without it.
-    const bool fused = lin && (ndim == 2 || tune.dc_fused3d);
+    S.it = e;
+    HIP_TRY(hipMemcpyAsync(&ctrl_h[S.seg & 1], ctrl_in, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipEventRecord(ev[S.seg & 1], stream));
+    return CDIFF_OK;
+  };
+  // the first segment runs to the check at or after the previous
+  // correction's sweep count, later ones one check each; one segment is
+  // always queued ahead of the host's read, so a correction that converges
+  // leaves at most one check's launches (which return at once) behind it
+  auto first_len = [&]() {
+    return (int)std::min<int64_t>(
+        INT_MAX / 2, std::max<int64_t>(ichk, (std::max<int64_t>(dc_inner_guess, 1) + ichk - 1) / ichk * ichk));
+  };
+  // The fused outer passes take the correction from the buffer that the
+  // solve's last check recorded (k_inner_finalize), so the pass is enqueued
+  // right behind the solve; behind the pass, the next solve's reset (done at
+  // once when the pass converged or failed) and first segment, so the GPU
+  // does not wait for the host between rounds.  The host reads the solve's
+  // and the pass's control blocks together; a solve that diverged is dropped
+  // and the pass redone without it (the next solve enqueued behind it is
+  // then stale and is restarted).
+  const bool fused = lin && (ndim == 2 || tune.dc_fused3d);
+  Solve S;
+  bool started = false;  // S's reset and first segment are already enqueued
+  while (!ctrl_h[2].done && !dc_failed) {
+    if (!started) {
+      if (outer >= max_outer) {
+        dc_failed = true;
+        break;
+      }
+      // inner FP32 solve for the correction, from e = 0 and q = 0 (the
+      // coefficient-form sweeps read no state on their first launch)
+      if (!lin) {
+        hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, u32[0]);
+        for (int ax = 0; ax < ndim; ++ax)
+          for (int b = 0; b < 2; ++b)
+            hipLaunchKernelGGL(k_fill32, grid_n(nq), dim3(256), 0, stream, nq, 0.0f, q32[ax][b]);
+      }
+      hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, stream, ctrl_in, 0.0, 0.0);
+      LAUNCH_CHECK("inner reset");
+      // (the coefficient-form sweep's coefficients come with the residual pass)
+      // the FP32 sweeps of a step never exceed max_iter: each correction solve
+      // gets what is left of it, and none is left once it is used up
+      const int64_t left = total_cap - inner_total;
+      if (left <= 0) {
+        dc_failed = true;
+        break;
+      }
+      S = Solve{};
+      S.cap = (int)std::min<int64_t>(fixed_inner > 0 ? fixed_inner : INT_MAX - 1, left);
+      if ((rc = enqueue(S, first_len()))) return rc;
+    }
+    started = false;
+    while (true) {
+      if (S.pending) {
+        HIP_TRY(hipEventSynchronize(ev[(S.seg - 1) & 1]));
+        if (dc_debug)
+          std::fprintf(stderr, "[dc]   inner check %d: residual %.3e\n", ctrl_h[(S.seg - 1) & 1].nchecks,
+                       ctrl_h[(S.seg - 1) & 1].last_l2);
+        if (ctrl_h[(S.seg - 1) & 1].done) break;
+      }
+      if (S.it >= S.cap) break;
+      S.pending = true;
+      ++S.seg;
+      if ((rc = enqueue(S, ichk))) return rc;
+    }
     const Ctrl outer_before = ctrl_h[2];  // the outer control block the pass starts from
     if (fused) {
       if ((rc = dc_outer_pass(dt, u[a], nullptr, u[1 - a], u_prev, outer + 2, o, true, ctrl_in))) return rc;
       HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
     }
     HIP_TRY(hipMemcpyAsync(&ctrl_h[3], ctrl_in, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
-    const int ni = ctrl_h[3].final_iter > 0 ? ctrl_h[3].final_iter : it;
+    HIP_TRY(hipEventRecord(ev[2], stream));
+    // the next solve behind the pass, where the iteration budget cannot bind
+    // within its first segment (so that it runs exactly as it would when
+    // enqueued after the host's read)
+    Solve N;
+    bool next = false;
+    if (fused && fixed_inner <= 0 && outer + 1 < max_outer) {
+      const int flen = first_len();
+      const int64_t left_next = total_cap - inner_total - S.it;  // S.it >= the solve's sweeps
+      if (left_next > (int64_t)flen + ichk) {
+        hipLaunchKernelGGL(k_ctrl_reset_follow, dim3(1), dim3(1), 0, stream, ctrl_in, (const Ctrl*)ctrl);
+        LAUNCH_CHECK("inner reset");
+        N.cap = (int)std::min<int64_t>(INT_MAX - 1, left_next);
+        if ((rc = enqueue(N, flen))) return rc;
+        next = true;
+      }
+    }
+    HIP_TRY(hipEventSynchronize(ev[2]));
+    const int ni = ctrl_h[3].final_iter > 0 ? ctrl_h[3].final_iter : S.it;
     dc_inner_guess = ni;
     inner_total += ni;
     inner_checks += ctrl_h[3].nchecks;
@@ -774,8 +813,8 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
       dc_dtau_scale_dflt = dc_theta_scale_dflt = 1.0;
       if (fused) {
         // the pass above applied the dropped correction: its control block back
-        ctrl_h[0] = outer_before;
-        HIP_TRY(hipMemcpyAsync(ctrl, &ctrl_h[0], sizeof(Ctrl), hipMemcpyHostToDevice, stream));
+        ctrl_h[3] = outer_before;
+        HIP_TRY(hipMemcpyAsync(ctrl, &ctrl_h[3], sizeof(Ctrl), hipMemcpyHostToDevice, stream));
       }
       if ((rc = dc_residual(dt, a, nullptr, 1 - a, outer + 2, o))) return rc;
       a = 1 - a;
@@ -785,8 +824,8 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     }
     ++outer;
     if (!fused) {
-      int eb = ends.empty() ? 0 : ends.back().second;
-      for (const auto& pr : ends)
+      int eb = S.ends.empty() ? 0 : S.ends.back().second;
+      for (const auto& pr : S.ends)
         if (pr.first == ni) eb = pr.second;
       if ((rc = dc_residual(dt, a, u32[eb], 1 - a, outer + 1, o))) return rc;
       HIP_TRY(hipMemcpyAsync(&ctrl_h[2], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
@@ -799,6 +838,12 @@ int cdiff_solver::step_fp32_dc(double dt, const cdiff_opts& o, double Vpdtau, cd
     if (failed_now()) {
       dc_failed = true;
       break;
+    }
+    if (next) {
+      // the next solve goes on from its first segment with the budget it has
+      N.cap = (int)std::min<int64_t>(INT_MAX - 1, total_cap - inner_total);
+      S = std::move(N);
+      started = true;
     }
   }
   Ctrl& c = ctrl_h[2];

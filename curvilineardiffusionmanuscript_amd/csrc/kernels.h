@@ -37,6 +37,8 @@ __global__ void k_merge(Geo g, const double* dom_src, const double* rest, double
 __global__ void k_fill(int64_t n, double v, double* dst);
 __global__ void k_validate(Geo g, const double* a, int positivity, int shift, Ctrl* ctrl);
 __global__ void k_ctrl_reset(Ctrl* ctrl, double rtol, double atol);
+// k_ctrl_reset(ctrl, 0, 0), done when the outer block `follow` is done or has failed
+__global__ void k_ctrl_reset_follow(Ctrl* ctrl, const Ctrl* follow);
 __global__ void k_ctrl_validated(Ctrl* ctrl);
 __global__ void k_finalize(Ctrl* ctrl, const double* psum, const double* pmax, int nparts, int iter, int max_iter,
                            double rtol, double atol, double nfull, int norm_kind, int check);

@@ -202,6 +202,21 @@ __global__ void k_ctrl_reset(Ctrl* ctrl, double rtol, double atol) {
   ctrl->ref = INFINITY;
 }
 
+__global__ void k_ctrl_reset_follow(Ctrl* ctrl, const Ctrl* follow) {
+  ctrl->done = (follow->done || follow->status != 0) ? 1 : 0;
+  ctrl->status = 0;
+  ctrl->final_iter = 0;
+  ctrl->nchecks = 0;
+  ctrl->rel = 0.0;
+  ctrl->abs_ = 0.0;
+  ctrl->init = INFINITY;
+  ctrl->last_l2 = 0.0;
+  ctrl->valid_flags = 0;
+  ctrl->grow = 0;
+  ctrl->ref_iter = 0;
+  ctrl->ref = INFINITY;
+}
+
 __global__ void k_ctrl_validated(Ctrl* ctrl) {
   const int f = ctrl->valid_flags;
   if (f & 0xffff) {

@@ -42,7 +42,7 @@ struct cdiff_solver {
   double* red_h = nullptr;
   cdiff::Ctrl* ctrl = nullptr;
   cdiff::Ctrl* ctrl_h = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // [0, 1]: segments read one behind; [2]: a correction round's end
   std::vector<void*> owned;
 
   // FP32 shadows (defect-correction mode)
