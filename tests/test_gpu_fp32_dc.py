@@ -324,3 +324,43 @@ def test_slowly_diverging_correction_solve_is_dropped_early():
     sol.close()
     print("inner sweeps per step", its)
     assert max(its) <= 600
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_budget_equal_to_the_sweeps_taken_changes_nothing(ndim):
+    """The next correction solve is enqueued behind the outer pass before the
+    host has read whether the round converged, and only where the step's
+    sweep budget (max_iter) cannot bind within its first segment.  A budget
+    equal to the sweeps a step takes without one must give the same sweeps,
+    rounds and bit-identical temperatures, step after step."""
+    if ndim == 2:
+        mesh, T0 = hot_disk_2d(512, seed=3)
+        law = S.PowerLawConductivity(1.0, 2.5, 0.0)
+        bcs = [S.DirichletBC(1.0), S.NeumannBC(), S.NeumannBC(), S.NeumannBC()]
+        dt0 = 1e-5
+    else:
+        mesh, T0 = gaussian_spots_3d(48, seed=2)
+        law = S.PowerLawConductivity(1.0, 3.0, 0.0)
+        bcs = [S.NeumannBC()] * 6
+        dt0 = 1e-4
+
+    def run(budgets):
+        sol = S.PseudoTransientSolver(mesh, bcs)
+        S.update_conductivity(sol, mesh, T0, 1.0, 1.0, law)
+        sol.set_temperature(T0)
+        dt, stats, fields = dt0, [], []
+        for b in budgets:
+            st, dt = sol.step_resident(dt, None, 1.0, law, rtol=0.0, atol=1e-9, max_iter=b, precision="fp32_dc")
+            assert not st.fp64_fallback and st.abs_err <= 1e-9
+            stats.append((st.niter, st.outer))
+            fields.append(np.array(sol.get_temperature(), copy=True))
+        sol.close()
+        return stats, fields
+
+    free, T_free = run([10**7] * 4)
+    assert max(o for _, o in free) >= 2, free  # several rounds: the next solve starts ahead of the host
+    tight, T_tight = run([n for n, _ in free])
+    print("sweeps, rounds per step", free)
+    assert tight == free
+    for a, b in zip(T_free, T_tight):
+        np.testing.assert_array_equal(a, b)
